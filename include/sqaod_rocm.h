@@ -1,0 +1,118 @@
+/*
+ * sqaod-rocm C ABI: the rocm device of sqaod's simulated-quantum-annealing
+ * solvers, for any FFI (the Python package binds it with ctypes).
+ *
+ * Each entry point replaces one function of the reference's per-device
+ * CPython extension (the method tables of sqaodc/pyglue/annealer.inc:885-908,
+ * bf_searcher.inc:479-494 and the device module sqaodpy/sqaod/cuda/src/
+ * cuda_device.cpp), with the same name stem and argument meaning; numpy
+ * arrays become plain pointers plus sizes, the dtype argument becomes the
+ * `dtype` given at creation (0 = float32, 1 = float64).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and -1 on error; the message of the
+ *    last error on the calling thread is returned by sqaod_rocm_last_error().
+ *    Errors are the reference's RuntimeError cases (pyglue.h:394-399): wrong
+ *    call order (state machine, Solver.cpp:96-112), shape mismatches, HIP
+ *    failures.
+ *  - Matrices are dense row-major without padding.  Spins q and bits x are
+ *    int8 (q in {-1, +1}, x in {0, 1}).
+ *  - Outputs are written into caller-provided buffers of the stated size.
+ *  - Solvers are single-owner objects: calls on one handle must not overlap.
+ */
+#ifndef SQAOD_ROCM_H
+#define SQAOD_ROCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sqaod_rocm_device sqaod_rocm_device;
+typedef struct sqaod_rocm_dg_annealer sqaod_rocm_dg_annealer;
+
+/* ---- library ---- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *sqaod_rocm_last_error(void);
+/* Library version; replaces sqaodc_cuda_version (sqaodc/cuda/api_cuda.cpp:5-9,
+ * probed by sqaodpy/sqaod/common/envcheck.py:15-62). */
+void sqaod_rocm_version(int *major, int *minor);
+/* Number of visible GPUs (sqaod.is_cuda_available counterpart, Common.cpp:20-77). */
+int sqaod_rocm_device_count(int *count);
+
+/* ---- device (cuda_device.cpp new/initialize/finalize/delete) ---- */
+int sqaod_rocm_device_new(int devno, sqaod_rocm_device **out);
+int sqaod_rocm_device_delete(sqaod_rocm_device *dev);
+int sqaod_rocm_device_synchronize(sqaod_rocm_device *dev);
+
+/* ---- dense graph annealer: annealer.inc method table (:885-908) ---- */
+
+/* annealer_new (:14-30) */
+int sqaod_rocm_dg_annealer_new(int dtype, sqaod_rocm_dg_annealer **out);
+/* annealer_delete (:33-51) */
+int sqaod_rocm_dg_annealer_delete(sqaod_rocm_dg_annealer *a);
+/* annealer_assign_device (:54-74); without it the default device 0 is used */
+int sqaod_rocm_dg_annealer_assign_device(sqaod_rocm_dg_annealer *a, sqaod_rocm_device *dev);
+/* annealer_seed (:77-105) */
+int sqaod_rocm_dg_annealer_seed(sqaod_rocm_dg_annealer *a, unsigned long long seed);
+/* annealer_set_qubo (:108-141): W is N x N, symmetric; optimize 0 = minimize, 1 = maximize */
+int sqaod_rocm_dg_annealer_set_qubo(sqaod_rocm_dg_annealer *a, const void *W, int N, int optimize);
+/* annealer_set_hamiltonian (:144-160): h[N], J[N x N] symmetric, c scalar (all of dtype) */
+int sqaod_rocm_dg_annealer_set_hamiltonian(sqaod_rocm_dg_annealer *a, const void *h, const void *J, const void *c,
+                                           int N);
+/* annealer_get_problem_size (:163-196) */
+int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
+/* annealer_set_preferences (:279-299), one preference per call, value as text:
+ * "algorithm" ("coloring" | "sa_naive" | "naive" | "default" | ...),
+ * "n_trotters" (int), and the rocm-device knobs "rng" ("philox" | "mt19937"),
+ * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2"),
+ * "slices_per_block" (int, 0 = auto), "refresh_interval" (int). */
+int sqaod_rocm_dg_annealer_set_preference(sqaod_rocm_dg_annealer *a, const char *name, const char *value);
+/* annealer_get_preferences (:302-329): "name=value" pairs joined by ';'.
+ * Writes at most buflen bytes (NUL-terminated); *needed receives the full length + 1. */
+int sqaod_rocm_dg_annealer_get_preferences(sqaod_rocm_dg_annealer *a, char *buf, size_t buflen, size_t *needed);
+/* number of Trotter slices m (the length of get_E / rows of get_x, get_q) */
+int sqaod_rocm_dg_annealer_get_n_trotters(sqaod_rocm_dg_annealer *a, int *m);
+/* annealer_get_E (:332-364): E[m] of dtype */
+int sqaod_rocm_dg_annealer_get_E(sqaod_rocm_dg_annealer *a, void *E, int m);
+/* annealer_get_x (:367-392): x[m x N] int8 */
+int sqaod_rocm_dg_annealer_get_x(sqaod_rocm_dg_annealer *a, int8_t *x, int m, int N);
+/* annealer_get_hamiltonian (:470-503): h[N], J[N x N], c[1] of dtype */
+int sqaod_rocm_dg_annealer_get_hamiltonian(sqaod_rocm_dg_annealer *a, void *h, void *J, void *c);
+/* annealer_get_q (:506-548): q[m x N] int8 */
+int sqaod_rocm_dg_annealer_get_q(sqaod_rocm_dg_annealer *a, int8_t *q, int m, int N);
+/* annealer_set_q (:395-431): one q[N] for every slice */
+int sqaod_rocm_dg_annealer_set_q(sqaod_rocm_dg_annealer *a, const int8_t *q, int N);
+/* annealer_set_qset (:434-467): q[m x N], sets m */
+int sqaod_rocm_dg_annealer_set_qset(sqaod_rocm_dg_annealer *a, const int8_t *q, int m, int N);
+/* annealer_radomize_q (:747-763) */
+int sqaod_rocm_dg_annealer_randomize_spin(sqaod_rocm_dg_annealer *a);
+/* annealer_calculate_E (:766-783) */
+int sqaod_rocm_dg_annealer_calculate_E(sqaod_rocm_dg_annealer *a);
+/* annealer_prepare (:786-802) */
+int sqaod_rocm_dg_annealer_prepare(sqaod_rocm_dg_annealer *a);
+/* annealer_make_solution (:805-833) */
+int sqaod_rocm_dg_annealer_make_solution(sqaod_rocm_dg_annealer *a);
+/* annealer_get_system_E (:836-850): G, beta converted to dtype; result as double */
+int sqaod_rocm_dg_annealer_get_system_E(sqaod_rocm_dg_annealer *a, double G, double beta, double *E);
+/* annealer_anneal_one_step (:853-878): G, beta converted to dtype */
+int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, double beta);
+/* Diagnostics without a reference counterpart: cumulative accepted flips,
+ * committed batches and blocked waits of the sweeps since prepare(), and the
+ * max |E_dtype - E_fp64| of the last energy evaluation. */
+int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]);
+int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err);
+
+/* Self check without a reference counterpart: order-independent checksum of
+ * the device exp (the acceptance test's expf) over float bit patterns
+ * [lo, hi), for comparison with the host C library's expf:
+ * sum of splitmix64((b << 32) | bits(expf(b))), NaN canonicalised. */
+int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsigned long long *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,670 @@
+/*
+ * rocm DenseGraphAnnealer<real>: the sqaodc dense annealer API
+ * (sqaodc/common/Solver.h:187-205) with device-resident spins, local fields
+ * and random streams.  Method semantics follow
+ * sqaodc/cpu/CPUDenseGraphAnnealer.cpp and sqaodc/cuda/CUDADenseGraphAnnealer.cu;
+ * the file:line of the behaviour each method mirrors is given at the method.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <strings.h>
+
+#include "solver.h"
+
+namespace sqaod_rocm {
+
+/* ---------------- shared helpers ---------------- */
+
+void logf(const char *fmt, ...) {
+    static int verbose = -1;
+    if (verbose < 0) {
+        const char *v = getenv("SQAOD_VERBOSE");
+        verbose = (v && atoi(v) != 0) ? 1 : 0;
+    }
+    if (!verbose)
+        return;
+    va_list ap;
+    va_start(ap, fmt);
+    vfprintf(stderr, fmt, ap);
+    va_end(ap);
+    fputc('\n', stderr);
+}
+
+const char *algorithmToString(Algorithm a) {
+    /* Preference.cpp:20-40 */
+    switch (a) {
+    case algoNaive: return "naive";
+    case algoColoring: return "coloring";
+    case algoBruteForceSearch: return "brute_force_search";
+    case algoSADefault: return "sa_default";
+    case algoSANaive: return "sa_naive";
+    case algoSAColoring: return "sa_coloring";
+    case algoDefault: return "default";
+    default: return "unknown";
+    }
+}
+
+Algorithm algorithmFromString(const char *s) {
+    /* Preference.cpp:42-58 */
+    if (!strcasecmp("naive", s)) return algoNaive;
+    if (!strcasecmp("coloring", s)) return algoColoring;
+    if (!strcasecmp("brute_force_search", s)) return algoBruteForceSearch;
+    if (!strcasecmp("sa_default", s)) return algoSADefault;
+    if (!strcasecmp("sa_naive", s)) return algoSANaive;
+    if (!strcasecmp("sa_coloring", s)) return algoSAColoring;
+    if (!strcasecmp("default", s)) return algoDefault;
+    return algoUnknown;
+}
+
+bool isSQAAlgorithm(Algorithm a) { return a == algoNaive || a == algoColoring; }
+
+void SolverState::clearState(int s) {
+    /* Solver.cpp:39-64: clearing a state clears every state that depends on it */
+    int toClear = 0;
+    switch (s) {
+    case solRandSeedGiven:
+        toClear = solRandSeedGiven;
+        break;
+    case solProblemSet:
+        toClear |= solProblemSet;
+        /* fall through */
+    case solPrepared:
+        toClear |= solPrepared;
+        /* fall through */
+    case solQSet:
+        toClear |= solQSet;
+        /* fall through */
+    case solEAvailable:
+    case solSolutionAvailable:
+        toClear |= solEAvailable | solSolutionAvailable;
+        break;
+    default:
+        break;
+    }
+    state_ &= ~toClear;
+}
+
+void SolverState::throwErrorIfProblemNotSet() const { SQ_THROW_IF(!isProblemSet(), "Problem is not set."); }
+
+void SolverState::throwErrorIfNotPrepared() const {
+    throwErrorIfProblemNotSet();
+    SQ_THROW_IF(!isPrepared(), "not prepared, call prepare() in advance.");
+}
+
+void SolverState::throwErrorIfQNotSet() const {
+    SQ_THROW_IF(!isQSet(), "Bits(x or q) not initialized.  Plase set or randomize in advance.");
+}
+
+Device::Device(int devno) : devno_(devno) {
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    SQ_THROW_IF(devno < 0 || devno >= n, "device %d not available (%d devices)", devno, n);
+    HIP_CHECK(hipSetDevice(devno));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, devno));
+    numCUs_ = prop.multiProcessorCount;
+    maxLds_ = prop.sharedMemPerBlock;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, devno) == hipSuccess && optin > 0)
+        maxLds_ = (size_t)optin;
+}
+
+Device::~Device() {
+    (void)hipSetDevice(devno_);
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+}
+
+void Device::makeCurrent() const { HIP_CHECK(hipSetDevice(devno_)); }
+void Device::synchronize() const { HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+Device &defaultDevice() {
+    static Device *dev = new Device(0); /* lives until process exit */
+    return *dev;
+}
+
+/* ---------------- DenseGraphAnnealer ---------------- */
+
+template <class real>
+DenseGraphAnnealer<real>::DenseGraphAnnealer() {
+    selectAlgorithm(algoDefault); /* CPUDenseGraphAnnealer.cpp:14-21 */
+}
+
+template <class real>
+DenseGraphAnnealer<real>::~DenseGraphAnnealer() {
+    if (dev_) {
+        (void)hipSetDevice(dev_->devno());
+        (void)hipStreamSynchronize(dev_->stream());
+    }
+    if (abortEvent_)
+        (void)hipEventDestroy(abortEvent_);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::assignDevice(Device &dev) {
+    /* cuda/api.h DeviceAssigner: a device is assigned once */
+    SQ_THROW_IF(dev_ != nullptr && dev_ != &dev, "Device assigned more than once.");
+    dev_ = &dev;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::ensureDevice() {
+    if (!dev_)
+        dev_ = &defaultDevice();
+    dev_->makeCurrent();
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::seed(unsigned long long s) {
+    /* :29-33 (the device streams are keyed by the seed; worker 0 = s) */
+    seed_ = s;
+    sweepIdx_ = 0;
+    if (d_mt_.ptr) {
+        ensureDevice();
+        MTState st;
+        mtSeedHost(&st, s);
+        HIP_CHECK(hipMemcpyAsync(d_mt_.ptr, &st, sizeof(MTState), hipMemcpyHostToDevice, dev_->stream()));
+        dev_->synchronize();
+    }
+    setState(solRandSeedGiven);
+}
+
+template <class real>
+Algorithm DenseGraphAnnealer<real>::selectAlgorithm(Algorithm algo) {
+    /* CUDADenseGraphAnnealer.cu:115-126: coloring and sa_naive are offered;
+     * other requests map to the default of their kind (Solver.cpp:152-168). */
+    switch (algo) {
+    case algoColoring:
+    case algoSANaive:
+        algo_ = algo;
+        break;
+    default: {
+        Algorithm prev = algo;
+        if (algo == algoDefault)
+            algo_ = algoColoring;
+        else if (algo == algoSADefault)
+            algo_ = algoSANaive;
+        else
+            algo_ = isSQAAlgorithm(algo) ? algoColoring : algoSANaive;
+        if (algo != algoDefault && algo != algoSADefault)
+            logf("%s is not supported, selecting the default algorithm of %s.", algorithmToString(prev),
+                 algorithmToString(algo_));
+        break;
+    }
+    }
+    return algo_;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std::string &value) {
+    /* Solver.cpp:139-150 for n_trotters/algorithm; the rest are rocm-device knobs */
+    if (name == "n_trotters") {
+        int m = atoi(value.c_str());
+        SQ_THROW_IF(m <= 0, "# trotters must be a positive integer.");
+        if (m_ != m)
+            clearState(solPrepared);
+        m_ = m;
+    } else if (name == "algorithm") {
+        Algorithm a = algorithmFromString(value.c_str());
+        if (a == algoUnknown)
+            a = (Algorithm)atoi(value.c_str());
+        selectAlgorithm(a);
+    } else if (name == "rng") {
+        if (value == "mt19937")
+            rngMT_ = true;
+        else if (value == "philox")
+            rngMT_ = false;
+        else
+            SQ_THROW("unknown rng '%s' (mt19937 | philox)", value.c_str());
+        clearState(solPrepared);
+    } else if (name == "sweep") {
+        if (value == "plain")
+            plainSweep_ = true;
+        else if (value == "local")
+            plainSweep_ = false;
+        else
+            SQ_THROW("unknown sweep '%s' (local | plain)", value.c_str());
+        fieldsDirty_ = true;
+    } else if (name == "field_precision") {
+        int f;
+        if (value == "fp32" || value == "fp64" || value == "native")
+            f = fieldF32;
+        else if (value == "bf16x2")
+            f = fieldBF16x2;
+        else
+            SQ_THROW("unknown field_precision '%s' (fp32 | bf16x2)", value.c_str());
+        SQ_THROW_IF(f == fieldBF16x2 && sizeof(real) != 4, "bf16x2 fields are available for float32 solvers");
+        if (f != fieldFormat_) {
+            fieldFormat_ = f;
+            fieldsDirty_ = true;
+            if (f == fieldBF16x2 && d_J_.ptr) {
+                ensureDevice();
+                d_Jhi_.allocate((size_t)N_ * Np_);
+                d_Jlo_.allocate((size_t)N_ * Np_);
+                splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, dev_->stream());
+            }
+        }
+    } else if (name == "slices_per_block") {
+        slicesPerBlock_ = atoi(value.c_str());
+    } else if (name == "refresh_interval") {
+        refreshInterval_ = atoi(value.c_str());
+    } else if (name == "spin_limit") {
+        spinLimit_ = atoi(value.c_str());
+    } else if (name == "precision" || name == "device") {
+        /* read-only (Preference.h:26-36) */
+    } else if (name == "experiment" || name == "tile_size") {
+        /* accepted and ignored, as the reference annealers do */
+    } else {
+        SQ_THROW("unknown preference '%s'", name.c_str());
+    }
+}
+
+template <class real>
+PrefList DenseGraphAnnealer<real>::getPreferences() const {
+    /* Solver.cpp:130-137 plus the device entry (CUDADenseGraphAnnealer.cu:179-183) */
+    PrefList p;
+    p.emplace_back("algorithm", algorithmToString(algo_));
+    p.emplace_back("n_trotters", std::to_string(m_));
+    p.emplace_back("precision", sizeof(real) == 4 ? "float" : "double");
+    p.emplace_back("device", "rocm");
+    p.emplace_back("rng", rngMT_ ? "mt19937" : "philox");
+    p.emplace_back("sweep", plainSweep_ ? "plain" : "local");
+    p.emplace_back("field_precision", fieldFormat_ == fieldBF16x2 ? "bf16x2" : (sizeof(real) == 4 ? "fp32" : "fp64"));
+    p.emplace_back("slices_per_block", std::to_string(slicesPerBlock_));
+    p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
+    return p;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::allocateProblem(int N) {
+    ensureDevice();
+    N_ = N;
+    Np_ = roundUp(N, 256);
+    Nw_ = Np_ / 32;
+    m_ = N / 4; /* CPUDenseGraphAnnealer.cpp:56, CUDADenseGraphAnnealer.cu:143 */
+    d_J_.allocate((size_t)N_ * Np_);
+    d_h_.allocate((size_t)Np_);
+    HIP_CHECK(hipMemsetAsync(d_J_.ptr, 0, d_J_.bytes(), dev_->stream()));
+    HIP_CHECK(hipMemsetAsync(d_h_.ptr, 0, d_h_.bytes(), dev_->stream()));
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::uploadJ(const real *J, const real *h, real c) {
+    HIP_CHECK(hipMemcpy2DAsync(d_J_.ptr, (size_t)Np_ * sizeof(real), J, (size_t)N_ * sizeof(real),
+                               (size_t)N_ * sizeof(real), N_, hipMemcpyHostToDevice, dev_->stream()));
+    HIP_CHECK(hipMemcpyAsync(d_h_.ptr, h, (size_t)N_ * sizeof(real), hipMemcpyHostToDevice, dev_->stream()));
+    if (fieldFormat_ == fieldBF16x2) {
+        d_Jhi_.allocate((size_t)N_ * Np_);
+        d_Jlo_.allocate((size_t)N_ * Np_);
+        splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, dev_->stream());
+    }
+    c_ = c;
+    h_.assign(h, h + N_);
+    dev_->synchronize();
+    fieldsDirty_ = true;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::setQUBO(const real *W, int N, OptimizeMethod om) {
+    /* CPUDenseGraphAnnealer.cpp:52-69 with DGFuncs::calculateHamiltonian
+     * (SharedFormulas.cpp:36-55): h = -1/2 colsum W, J = -1/4 W, c = sum J +
+     * trace J, diag J = 0; maximize negates h, J, c.  Sums are accumulated in
+     * double and rounded once. */
+    SQ_THROW_IF(N <= 0, "W must be a non-empty square matrix.");
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < i; ++j)
+            SQ_THROW_IF(W[(size_t)i * N + j] != W[(size_t)j * N + i], "W is not symmetric.");
+    clearState(solProblemSet);
+    allocateProblem(N);
+    om_ = om;
+    const real sgn = (om == optMaximize) ? real(-1.) : real(1.);
+    std::vector<real> h(N), J((size_t)N * N);
+    double jsum = 0., dsum = 0.;
+    for (int j = 0; j < N; ++j) {
+        double s = 0.;
+        for (int i = 0; i < N; ++i)
+            s += (double)W[(size_t)i * N + j];
+        h[j] = sgn * (real)(-0.5 * s);
+    }
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            real v = real(-0.25) * W[(size_t)i * N + j];
+            jsum += (double)v;
+            if (i == j)
+                dsum += (double)v;
+            J[(size_t)i * N + j] = (i == j) ? real(0.) : sgn * v;
+        }
+    uploadJ(J.data(), h.data(), sgn * (real)(jsum + dsum));
+    setState(solProblemSet);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::setHamiltonian(const real *h, const real *J, real c, int N) {
+    /* CPUDenseGraphAnnealer.cpp:71-84 (CUDA also requires symmetric J) */
+    SQ_THROW_IF(N <= 0, "J must be a non-empty square matrix.");
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < i; ++j)
+            SQ_THROW_IF(J[(size_t)i * N + j] != J[(size_t)j * N + i], "J is not symmetric.");
+    clearState(solProblemSet);
+    allocateProblem(N);
+    om_ = optMinimize;
+    /* J is kept as given, diagonal included: the reference's per-attempt dot
+     * includes J[x][x] q[x] (CPUDenseGraphAnnealer.cpp:257), and so do the
+     * local fields and their row updates here. */
+    uploadJ(J, h, c);
+    setState(solProblemSet);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::getHamiltonian(real *h, real *J, real *c) const {
+    throwErrorIfProblemNotSet();
+    dev_->makeCurrent();
+    memcpy(h, h_.data(), (size_t)N_ * sizeof(real));
+    HIP_CHECK(hipMemcpy2DAsync(J, (size_t)N_ * sizeof(real), d_J_.ptr, (size_t)Np_ * sizeof(real),
+                               (size_t)N_ * sizeof(real), N_, hipMemcpyDeviceToHost, dev_->stream()));
+    dev_->synchronize();
+    *c = c_;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::prepare() {
+    /* CUDADenseGraphAnnealer.cu:275-336 / CPUDenseGraphAnnealer.cpp:164-195 */
+    throwErrorIfProblemNotSet();
+    SQ_THROW_IF(m_ <= 0, "# trotters must be a positive integer.");
+    ensureDevice();
+    if (!isRandSeedGiven())
+        seed((unsigned long long)time(NULL));
+    setState(solRandSeedGiven);
+    if (m_ == 1 && algo_ != algoSANaive) {
+        algo_ = algoSANaive; /* selectDefaultSAAlgorithm, Solver.cpp:170-176 */
+        logf("Selecting %s as the default SA algorithm.", algorithmToString(algo_));
+    }
+    const size_t mN = (size_t)m_ * N_;
+    d_Q_.allocate((size_t)m_ * Nw_);
+    d_F_.allocate((size_t)m_ * Np_);
+    d_X_.allocate(mN);
+    d_U_.allocate(mN);
+    d_D_.allocate(mN);
+    HIP_CHECK(hipMemsetAsync(d_Q_.ptr, 0, d_Q_.bytes(), dev_->stream()));
+    HIP_CHECK(hipMemsetAsync(d_D_.ptr, 0, d_D_.bytes(), dev_->stream()));
+    tag_ = 0;
+    d_E_.allocate((size_t)m_);
+    d_partial_.allocate((size_t)m_ * (Np_ / 256));
+    d_spinDot_.allocate(1);
+    d_abort_.allocate(1);
+    d_counters_.allocate(3);
+    HIP_CHECK(hipMemsetAsync(d_abort_.ptr, 0, sizeof(int), dev_->stream()));
+    HIP_CHECK(hipMemsetAsync(d_counters_.ptr, 0, 3 * sizeof(unsigned long long), dev_->stream()));
+    h_abort_.allocate(1);
+    h_abort_.ptr[0] = 0;
+    if (!abortEvent_)
+        HIP_CHECK(hipEventCreateWithFlags(&abortEvent_, hipEventDisableTiming));
+    abortPending_ = false;
+    if (rngMT_) {
+        d_mt_.allocate(1);
+        MTState st;
+        mtSeedHost(&st, seed_);
+        HIP_CHECK(hipMemcpyAsync(d_mt_.ptr, &st, sizeof(MTState), hipMemcpyHostToDevice, dev_->stream()));
+        const int perAttempt = sizeof(real) == 4 ? 2 : 3;
+        d_words_.allocate(mN * perAttempt);
+    }
+    E_.assign((size_t)m_, real(0.));
+    q_.assign(mN, 0);
+    dev_->synchronize();
+    fieldsDirty_ = true;
+    setState(solPrepared);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::randomizeSpin() {
+    /* CPUDenseGraphAnnealer.cpp:151-161: q = randInt(2) ? +1 : -1, row-major */
+    throwErrorIfNotPrepared();
+    ensureDevice();
+    if (rngMT_) {
+        mtGenerate(d_mt_.ptr, d_words_.ptr, (size_t)m_ * N_, dev_->stream());
+        spinsFromWords(d_words_.ptr, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
+    } else {
+        spinsFromPhilox(seed_, sweepIdx_++, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
+    }
+    fieldsDirty_ = true;
+    setState(solQSet);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::set_q(const int8_t *q, int N) {
+    /* CUDADenseGraphAnnealer.cu:200-211: one q broadcast to every slice */
+    throwErrorIfNotPrepared();
+    SQ_THROW_IF(N != N_, "Dimension of q, %d, should be equal to N, %d.", N, N_);
+    ensureDevice();
+    std::vector<int8_t> all((size_t)m_ * N_);
+    for (int y = 0; y < m_; ++y)
+        memcpy(&all[(size_t)y * N_], q, (size_t)N_);
+    d_q8_.allocate(all.size());
+    HIP_CHECK(hipMemcpyAsync(d_q8_.ptr, all.data(), all.size(), hipMemcpyHostToDevice, dev_->stream()));
+    packSpins(d_q8_.ptr, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
+    dev_->synchronize();
+    fieldsDirty_ = true;
+    setState(solQSet);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::set_qset(const int8_t *q, int m, int N) {
+    /* CUDADenseGraphAnnealer.cu:213-230: m = number of vectors, prepare(), copy */
+    throwErrorIfProblemNotSet();
+    SQ_THROW_IF(N != N_, "Dimension of q, %d, should be equal to N, %d.", N, N_);
+    SQ_THROW_IF(m <= 0, "empty q set");
+    m_ = m;
+    prepare();
+    d_q8_.allocate((size_t)m_ * N_);
+    HIP_CHECK(hipMemcpyAsync(d_q8_.ptr, q, (size_t)m_ * N_, hipMemcpyHostToDevice, dev_->stream()));
+    packSpins(d_q8_.ptr, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
+    dev_->synchronize();
+    fieldsDirty_ = true;
+    setState(solQSet);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::checkAbort(bool synchronous) {
+    if (!abortPending_)
+        return;
+    if (synchronous)
+        HIP_CHECK(hipEventSynchronize(abortEvent_));
+    else if (hipEventQuery(abortEvent_) != hipSuccess)
+        return;
+    abortPending_ = false;
+    if (h_abort_.ptr[0]) {
+        clearState(solQSet);
+        SQ_THROW("sweep aborted: a slice waited past the spin limit on a neighbour's decision; spins are invalid");
+    }
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::refreshFields() {
+    if (plainSweep_) {
+        fieldsDirty_ = false;
+        return;
+    }
+    if (fieldFormat_ == fieldBF16x2)
+        fieldRefreshBF16x2(d_Jhi_.ptr, d_Jlo_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr, dev_->stream());
+    else
+        fieldRefresh<real>(d_J_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, d_F_.ptr, dev_->stream());
+    fieldsDirty_ = false;
+    stepsSinceRefresh_ = 0;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::generateAttempts() {
+    const int order = (algo_ == algoSANaive) ? orderSliceMajor : orderColoring;
+    if (rngMT_) {
+        const int perAttempt = sizeof(real) == 4 ? 2 : 3;
+        mtGenerate(d_mt_.ptr, d_words_.ptr, (size_t)m_ * N_ * perAttempt, dev_->stream());
+        streamToAttempts<real>(d_words_.ptr, N_, m_, order, d_X_.ptr, d_U_.ptr, dev_->stream());
+    } else {
+        philoxAttempts<real>(seed_, sweepIdx_++, N_, m_, d_X_.ptr, d_U_.ptr, dev_->stream());
+    }
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
+    /* CPUDenseGraphAnnealer.cpp:281-301 (coloring) and :354-371 (SA) */
+    throwErrorIfQNotSet();
+    ensureDevice();
+    checkAbort(false);
+    if (!fieldsDirty_ && refreshInterval_ > 0 && stepsSinceRefresh_ >= refreshInterval_)
+        fieldsDirty_ = true;
+    if (fieldsDirty_)
+        refreshFields();
+    generateAttempts();
+
+    SweepArgs<real> a{};
+    a.N = N_;
+    a.Np = Np_;
+    a.m = m_;
+    a.Nw = Nw_;
+    a.sa = (algo_ == algoSANaive) ? 1 : 0;
+    /* host scalars in the reference's expression order, in real (:283-284, :358) */
+    a.twoDivM = real(2.) / real(m_);
+    a.coef = std::log(std::tanh(G * beta / real(m_))) / beta;
+    a.beta = beta;
+    a.invKT = real(1.) / G;
+    a.J = d_J_.ptr;
+    a.Jhi = d_Jhi_.ptr;
+    a.Jlo = d_Jlo_.ptr;
+    a.h = d_h_.ptr;
+    a.F = d_F_.ptr;
+    a.Qbits = d_Q_.ptr;
+    a.X = d_X_.ptr;
+    a.U = d_U_.ptr;
+    a.D = d_D_.ptr;
+    a.tag = ++tag_;
+    if (tag_ >= 0x7fffffffu) {
+        tag_ = 1;
+        a.tag = 1;
+        HIP_CHECK(hipMemsetAsync(d_D_.ptr, 0, d_D_.bytes(), dev_->stream()));
+    }
+    a.abortFlag = d_abort_.ptr;
+    a.counters = d_counters_.ptr;
+    a.spinLimit = spinLimit_;
+
+    int S = slicesPerBlock_;
+    if (S <= 0) {
+        S = 8;
+        while (S > 1 && (denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_) > dev_->maxLdsPerBlock() || S > m_))
+            S >>= 1;
+    }
+    SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_) > dev_->maxLdsPerBlock(),
+                "N=%d too large for the LDS-resident sweep (%zu bytes needed per block)", N_,
+                denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_));
+    denseSweep<real>(a, plainSweep_, fieldFormat_, S, dev_->stream());
+
+    HIP_CHECK(hipMemcpyAsync(h_abort_.ptr, d_abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, dev_->stream()));
+    HIP_CHECK(hipEventRecord(abortEvent_, dev_->stream()));
+    abortPending_ = true;
+    ++stepsSinceRefresh_;
+    clearState(solSolutionAvailable);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::calculate_E() {
+    /* CUDADenseGraphAnnealer.cu:254-271; energies recomputed in FP64 */
+    throwErrorIfQNotSet();
+    ensureDevice();
+    energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
+                     dev_->stream());
+    std::vector<double> e((size_t)m_);
+    HIP_CHECK(hipMemcpyAsync(e.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost, dev_->stream()));
+    dev_->synchronize();
+    checkAbort(true);
+    const real sgn = (om_ == optMaximize) ? real(-1.) : real(1.);
+    lastERoundErr_ = 0.;
+    E_.resize((size_t)m_);
+    for (int y = 0; y < m_; ++y) {
+        E_[y] = sgn * (real)e[y];
+        double err = std::fabs((double)(real)e[y] - e[y]);
+        if (err > lastERoundErr_)
+            lastERoundErr_ = err;
+    }
+    setState(solEAvailable);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::makeSolution() {
+    /* CPUDenseGraphAnnealer.cpp:197-227: syncBits then calculate_E */
+    throwErrorIfQNotSet();
+    ensureDevice();
+    d_q8_.allocate((size_t)m_ * N_);
+    unpackSpins(d_Q_.ptr, N_, m_, Nw_, d_q8_.ptr, dev_->stream());
+    q_.resize((size_t)m_ * N_);
+    HIP_CHECK(hipMemcpyAsync(q_.data(), d_q8_.ptr, q_.size(), hipMemcpyDeviceToHost, dev_->stream()));
+    dev_->synchronize();
+    checkAbort(true);
+    setState(solSolutionAvailable);
+    calculate_E();
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::get_q(int8_t *q) {
+    if (!isSolutionAvailable())
+        makeSolution();
+    memcpy(q, q_.data(), q_.size());
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::get_x(int8_t *x) {
+    /* x = (q + 1) / 2, Common.cpp:133-145 */
+    if (!isSolutionAvailable())
+        makeSolution();
+    for (size_t i = 0; i < q_.size(); ++i)
+        x[i] = (int8_t)((q_[i] + 1) / 2);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::get_E(real *E) {
+    if (!isEAvailable())
+        calculate_E();
+    memcpy(E, E_.data(), (size_t)m_ * sizeof(real));
+}
+
+template <class real>
+real DenseGraphAnnealer<real>::getSystemE(real G, real beta) {
+    /* CPUDenseGraphAnnealer.cpp:230-247 */
+    calculate_E();
+    real E = real(0.);
+    for (int y = 0; y < m_; ++y)
+        E += E_[y];
+    E = E / real(m_);
+    if (isSQAAlgorithm(algo_)) {
+        double dot = 0.;
+        spinDotSum(d_Q_.ptr, N_, m_, Nw_, d_spinDot_.ptr, dev_->stream());
+        HIP_CHECK(hipMemcpyAsync(&dot, d_spinDot_.ptr, sizeof(double), hipMemcpyDeviceToHost, dev_->stream()));
+        dev_->synchronize();
+        real spinDotSum_ = (real)dot;
+        real coef = real(0.5) / beta * std::log(std::tanh(G * beta / real(m_)));
+        E -= spinDotSum_ * coef;
+    }
+    if (om_ == optMaximize)
+        E *= real(-1.);
+    return E;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::getCounters(unsigned long long out[3]) {
+    out[0] = out[1] = out[2] = 0;
+    if (!d_counters_.ptr)
+        return;
+    ensureDevice();
+    HIP_CHECK(hipMemcpyAsync(out, d_counters_.ptr, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             dev_->stream()));
+    dev_->synchronize();
+}
+
+template class DenseGraphAnnealer<float>;
+template class DenseGraphAnnealer<double>;
+
+}  // namespace sqaod_rocm

@@ -1,0 +1,188 @@
+/*
+ * Host-side solver objects of the rocm device.  They mirror the reference's
+ * abstract solver API (sqaodc/common/Solver.h:17-252: Solver / Annealer /
+ * DenseGraphAnnealer / BipartiteGraphAnnealer / DenseGraphBFSearcher<real>)
+ * and its state machine (sqaodc/common/Solver.cpp:32-112), with device-
+ * resident state driven through the HIP kernels of this directory.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace sqaod_rocm {
+
+/* sqaodc/common/Preference.h:8-17 */
+enum Algorithm {
+    algoUnknown = 0,
+    algoDefault = 1,
+    algoNaive = 2,
+    algoColoring = 3,
+    algoBruteForceSearch = 4,
+    algoSADefault = 5,
+    algoSANaive = 6,
+    algoSAColoring = 7,
+};
+const char *algorithmToString(Algorithm a);
+Algorithm algorithmFromString(const char *s);
+bool isSQAAlgorithm(Algorithm a);
+
+/* sqaodc/common/Solver.h:9-14 */
+enum OptimizeMethod { optMinimize = 0, optMaximize = 1 };
+
+/* Logging to stderr when SQAOD_VERBOSE is non-zero (defines.cpp:46-63). */
+void logf(const char *fmt, ...);
+
+/* A GPU and the stream its solvers enqueue on (cuda/Device.cpp role). */
+class Device {
+public:
+    explicit Device(int devno);
+    ~Device();
+    int devno() const { return devno_; }
+    hipStream_t stream() const { return stream_; }
+    int numCUs() const { return numCUs_; }
+    size_t maxLdsPerBlock() const { return maxLds_; }
+    void makeCurrent() const;
+    void synchronize() const;
+
+private:
+    int devno_;
+    hipStream_t stream_;
+    int numCUs_ = 0;
+    size_t maxLds_ = 65536;
+};
+
+/* The process-wide default device (device 0), created on first use, like
+ * sqaod.cuda's active_device (sqaodpy/sqaod/cuda/device.py:26-37). */
+Device &defaultDevice();
+
+/* Solver state machine, Solver.cpp:32-112. */
+class SolverState {
+public:
+    enum {
+        solNone = 0,
+        solProblemSet = 1,
+        solPrepared = 2,
+        solEAvailable = 4,
+        solSolutionAvailable = 8,
+        solRandSeedGiven = 16,
+        solQSet = 32,
+    };
+    void setState(int s) {
+        clearState(s);
+        state_ |= s;
+    }
+    void clearState(int s);
+    bool isRandSeedGiven() const { return state_ & solRandSeedGiven; }
+    bool isProblemSet() const { return state_ & solProblemSet; }
+    bool isPrepared() const { return state_ & solPrepared; }
+    bool isQSet() const { return state_ & solQSet; }
+    bool isEAvailable() const { return state_ & solEAvailable; }
+    bool isSolutionAvailable() const { return state_ & solSolutionAvailable; }
+    void throwErrorIfProblemNotSet() const;
+    void throwErrorIfNotPrepared() const;
+    void throwErrorIfQNotSet() const;
+
+protected:
+    int state_ = solNone;
+};
+
+using PrefList = std::vector<std::pair<std::string, std::string>>;
+
+/* ------------------------------------------------------------------ */
+/* Dense-graph annealer (CPUDenseGraphAnnealer / CUDADenseGraphAnnealer)   */
+/* ------------------------------------------------------------------ */
+template <class real>
+class DenseGraphAnnealer : public SolverState {
+public:
+    DenseGraphAnnealer();
+    ~DenseGraphAnnealer();
+
+    void assignDevice(Device &dev);
+
+    void seed(unsigned long long s);
+    Algorithm selectAlgorithm(Algorithm algo);
+    Algorithm getAlgorithm() const { return algo_; }
+    void setPreference(const std::string &name, const std::string &value);
+    PrefList getPreferences() const;
+
+    void setQUBO(const real *W, int N, OptimizeMethod om);
+    void setHamiltonian(const real *h, const real *J, real c, int N);
+    void getHamiltonian(real *h, real *J, real *c) const;
+    int getProblemSize() const { return N_; }
+    int numTrotters() const { return m_; }
+
+    void set_q(const int8_t *q, int N);
+    void set_qset(const int8_t *q, int m, int N);
+    void get_q(int8_t *q);             /* m x N */
+    void get_x(int8_t *x);             /* m x N */
+    void get_E(real *E);               /* m */
+    void randomizeSpin();
+    void prepare();
+    void calculate_E();
+    void makeSolution();
+    real getSystemE(real G, real beta);
+    void annealOneStep(real G, real beta);
+
+    /* diagnostics (not in the reference API) */
+    void getCounters(unsigned long long out[3]);
+    /* Max |E_real - E_fp64| of the last calculate_E (the rounding of the
+     * FP64 recompute to real). */
+    double lastEnergyRoundingError() const { return lastERoundErr_; }
+
+private:
+    void ensureDevice();
+    void checkAbort(bool synchronous);
+    void allocateProblem(int N);
+    void refreshFields();
+    void generateAttempts();
+    void uploadJ(const real *J, const real *h, real c);
+
+    Device *dev_ = nullptr;
+    int N_ = 0, Np_ = 0, Nw_ = 0, m_ = -1;
+    Algorithm algo_ = algoColoring;
+    OptimizeMethod om_ = optMinimize;
+    unsigned long long seed_ = 0;
+    unsigned long long sweepIdx_ = 0;
+    uint32_t tag_ = 0;
+    real c_ = 0;
+
+    /* preferences beyond the reference's */
+    bool rngMT_ = false;          /* "rng": "philox" (default) | "mt19937" (CPU-stream replay) */
+    bool plainSweep_ = false;     /* "sweep": "local" (default) | "plain" */
+    int fieldFormat_ = fieldF32;  /* "field_precision": "fp32" | "bf16x2" */
+    int slicesPerBlock_ = 0;      /* "slices_per_block": 0 = automatic */
+    int refreshInterval_ = 0;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
+    int spinLimit_ = 1 << 20;
+
+    bool fieldsDirty_ = true;
+    int stepsSinceRefresh_ = 0;
+    double lastERoundErr_ = 0.;
+
+    /* device state */
+    DevBuf<real> d_J_, d_h_, d_F_, d_U_;
+    DevBuf<uint16_t> d_Jhi_, d_Jlo_;
+    DevBuf<uint32_t> d_Q_, d_X_, d_D_, d_words_;
+    DevBuf<MTState> d_mt_;
+    DevBuf<double> d_E_, d_partial_, d_spinDot_;
+    DevBuf<int> d_abort_;
+    DevBuf<unsigned long long> d_counters_;
+    DevBuf<int8_t> d_q8_;
+    PinnedBuf<int> h_abort_;
+    hipEvent_t abortEvent_ = nullptr;
+    bool abortPending_ = false;
+
+    /* host copies */
+    std::vector<real> h_;     /* host h (for getHamiltonian) */
+    std::vector<real> E_;
+    std::vector<int8_t> q_;   /* m x N after makeSolution */
+};
+
+}  // namespace sqaod_rocm

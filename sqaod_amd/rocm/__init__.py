@@ -1,0 +1,5 @@
+"""The rocm device package, the counterpart of sqaod.cuda
+(sqaodpy/sqaod/cuda/__init__.py)."""
+from . import _cext  # noqa: F401  (fails loudly when libsqaod_rocm.so is missing)
+from .dense_graph_annealer import dense_graph_annealer, DenseGraphAnnealer  # noqa: F401
+from .device import create_device, Device  # noqa: F401

@@ -1,0 +1,267 @@
+"""ctypes binding of libsqaod_rocm.so (include/sqaod_rocm.h).
+
+Exposes per-solver namespaces whose function names and argument tuples are
+those of the reference's CPython extension modules (the method table of
+sqaodc/pyglue/annealer.inc:885-908), so the solver classes call
+``cext.anneal_one_step(cobj, G, beta, dtype)`` exactly as the reference's
+``DenseGraphAnnealerBase`` does.  The library is required: importing this
+module fails loudly when it has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libsqaod_rocm.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libsqaod_rocm.so not found at {LIB_PATH}; build it with `python -m sqaod_amd.build`")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_vp, _i32, _u64, _f64, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_size_t
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+_SIGS = {
+    "sqaod_rocm_last_error": (ctypes.c_char_p, []),
+    "sqaod_rocm_version": (None, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaod_rocm_device_count": (_i32, [ctypes.POINTER(_i32)]),
+    "sqaod_rocm_device_new": (_i32, [_i32, _PP]),
+    "sqaod_rocm_device_delete": (_i32, [_vp]),
+    "sqaod_rocm_device_synchronize": (_i32, [_vp]),
+    "sqaod_rocm_dg_annealer_new": (_i32, [_i32, _PP]),
+    "sqaod_rocm_dg_annealer_delete": (_i32, [_vp]),
+    "sqaod_rocm_dg_annealer_assign_device": (_i32, [_vp, _vp]),
+    "sqaod_rocm_dg_annealer_seed": (_i32, [_vp, _u64]),
+    "sqaod_rocm_dg_annealer_set_qubo": (_i32, [_vp, _vp, _i32, _i32]),
+    "sqaod_rocm_dg_annealer_set_hamiltonian": (_i32, [_vp, _vp, _vp, _vp, _i32]),
+    "sqaod_rocm_dg_annealer_get_problem_size": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dg_annealer_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "sqaod_rocm_dg_annealer_get_preferences": (_i32, [_vp, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
+    "sqaod_rocm_dg_annealer_get_n_trotters": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dg_annealer_get_E": (_i32, [_vp, _vp, _i32]),
+    "sqaod_rocm_dg_annealer_get_x": (_i32, [_vp, _vp, _i32, _i32]),
+    "sqaod_rocm_dg_annealer_get_hamiltonian": (_i32, [_vp, _vp, _vp, _vp]),
+    "sqaod_rocm_dg_annealer_get_q": (_i32, [_vp, _vp, _i32, _i32]),
+    "sqaod_rocm_dg_annealer_set_q": (_i32, [_vp, _vp, _i32]),
+    "sqaod_rocm_dg_annealer_set_qset": (_i32, [_vp, _vp, _i32, _i32]),
+    "sqaod_rocm_dg_annealer_randomize_spin": (_i32, [_vp]),
+    "sqaod_rocm_dg_annealer_calculate_E": (_i32, [_vp]),
+    "sqaod_rocm_dg_annealer_prepare": (_i32, [_vp]),
+    "sqaod_rocm_dg_annealer_make_solution": (_i32, [_vp]),
+    "sqaod_rocm_dg_annealer_get_system_E": (_i32, [_vp, _f64, _f64, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
+    "sqaod_rocm_dg_annealer_get_counters": (_i32, [_vp, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+EXPORTED = tuple(_SIGS)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(_lib.sqaod_rocm_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+def _dt(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return 1
+    if dt == np.float32:
+        return 0
+    raise RuntimeError("dtype must be np.float32 or np.float64")
+
+
+def version():
+    a, b = _i32(), _i32()
+    _lib.sqaod_rocm_version(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def device_count():
+    n = _i32()
+    _check(_lib.sqaod_rocm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def expf_checksum(lo=0, hi=1 << 32):
+    """Device-side checksum of the acceptance-test exp over float bit patterns [lo, hi)."""
+    out = _u64()
+    _check(_lib.sqaod_rocm_expf_checksum(lo, hi, ctypes.byref(out)))
+    return out.value
+
+
+class _Device:
+    """rocm_device module (cuda_device.cpp counterpart)."""
+
+    @staticmethod
+    def new(devno=0):
+        out = _vp()
+        _check(_lib.sqaod_rocm_device_new(devno, ctypes.byref(out)))
+        return out.value
+
+    @staticmethod
+    def delete(cobj):
+        _check(_lib.sqaod_rocm_device_delete(cobj))
+
+    @staticmethod
+    def synchronize(cobj):
+        _check(_lib.sqaod_rocm_device_synchronize(cobj))
+
+
+class _DenseGraphAnnealer:
+    """rocm_dg_annealer module: annealer.inc:885-908 method table."""
+
+    @staticmethod
+    def new(dtype):
+        out = _vp()
+        _check(_lib.sqaod_rocm_dg_annealer_new(_dt(dtype), ctypes.byref(out)))
+        return out.value
+
+    @staticmethod
+    def delete(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_delete(cobj))
+
+    @staticmethod
+    def assign_device(cobj, devobj, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_assign_device(cobj, devobj))
+
+    @staticmethod
+    def seed(cobj, seed, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_seed(cobj, int(seed) & 0xffffffffffffffff))
+
+    @staticmethod
+    def set_qubo(cobj, W, optimize, dtype):
+        W = np.ascontiguousarray(W, dtype)
+        if W.ndim != 2 or W.shape[0] != W.shape[1]:
+            raise RuntimeError("W must be a square matrix.")
+        _check(_lib.sqaod_rocm_dg_annealer_set_qubo(cobj, _ptr(W), W.shape[0], int(optimize)))
+
+    @staticmethod
+    def set_hamiltonian(cobj, h, J, c, dtype):
+        h = np.ascontiguousarray(h, dtype)
+        J = np.ascontiguousarray(J, dtype)
+        cc = np.array([c], dtype)
+        if J.ndim != 2 or J.shape[0] != J.shape[1] or h.shape != (J.shape[0],):
+            raise RuntimeError("shape mismatch between h and J.")
+        _check(_lib.sqaod_rocm_dg_annealer_set_hamiltonian(cobj, _ptr(h), _ptr(J), _ptr(cc), J.shape[0]))
+
+    @staticmethod
+    def get_problem_size(cobj, dtype):
+        n = _i32()
+        _check(_lib.sqaod_rocm_dg_annealer_get_problem_size(cobj, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def set_preferences(cobj, prefs, dtype):
+        for k, v in (prefs or {}).items():
+            if hasattr(v, '__int__') and not isinstance(v, (str, float)):
+                v = int(v)
+            _check(_lib.sqaod_rocm_dg_annealer_set_preference(cobj, str(k).encode(), str(v).encode()))
+
+    @staticmethod
+    def get_preferences(cobj, dtype):
+        need = _sz()
+        _check(_lib.sqaod_rocm_dg_annealer_get_preferences(cobj, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(_lib.sqaod_rocm_dg_annealer_get_preferences(cobj, buf, need.value, ctypes.byref(need)))
+        out = {}
+        for item in buf.value.decode().split(';'):
+            if not item:
+                continue
+            k, v = item.split('=', 1)
+            out[k] = int(v) if k in ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval') else v
+        return out
+
+    @staticmethod
+    def _m(cobj):
+        m = _i32()
+        _check(_lib.sqaod_rocm_dg_annealer_get_n_trotters(cobj, ctypes.byref(m)))
+        return m.value
+
+    @staticmethod
+    def get_E(cobj, dtype):
+        m = _DenseGraphAnnealer._m(cobj)
+        E = np.empty(m, dtype)
+        _check(_lib.sqaod_rocm_dg_annealer_get_E(cobj, _ptr(E), m))
+        return E
+
+    @staticmethod
+    def get_x(cobj, dtype):
+        m, N = _DenseGraphAnnealer._m(cobj), _DenseGraphAnnealer.get_problem_size(cobj, dtype)
+        x = np.empty((m, N), np.int8)
+        _check(_lib.sqaod_rocm_dg_annealer_get_x(cobj, _ptr(x), m, N))
+        return [row.copy() for row in x]
+
+    @staticmethod
+    def get_hamiltonian(cobj, h, J, c, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_get_hamiltonian(cobj, _ptr(h), _ptr(J), _ptr(c)))
+
+    @staticmethod
+    def get_q(cobj, dtype):
+        m, N = _DenseGraphAnnealer._m(cobj), _DenseGraphAnnealer.get_problem_size(cobj, dtype)
+        q = np.empty((m, N), np.int8)
+        _check(_lib.sqaod_rocm_dg_annealer_get_q(cobj, _ptr(q), m, N))
+        return [row.copy() for row in q]
+
+    @staticmethod
+    def set_q(cobj, q, dtype):
+        q = np.ascontiguousarray(q, np.int8).reshape(-1)
+        _check(_lib.sqaod_rocm_dg_annealer_set_q(cobj, _ptr(q), q.shape[0]))
+
+    @staticmethod
+    def set_qset(cobj, qlist, dtype):
+        q = np.ascontiguousarray(np.stack([np.asarray(v, np.int8).reshape(-1) for v in qlist]))
+        _check(_lib.sqaod_rocm_dg_annealer_set_qset(cobj, _ptr(q), q.shape[0], q.shape[1]))
+
+    @staticmethod
+    def randomize_spin(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_randomize_spin(cobj))
+
+    @staticmethod
+    def calculate_E(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_calculate_E(cobj))
+
+    @staticmethod
+    def prepare(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_prepare(cobj))
+
+    @staticmethod
+    def make_solution(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_make_solution(cobj))
+
+    @staticmethod
+    def get_system_E(cobj, G, beta, dtype):
+        E = _f64()
+        _check(_lib.sqaod_rocm_dg_annealer_get_system_E(cobj, float(G), float(beta), ctypes.byref(E)))
+        return np.dtype(dtype).type(E.value)
+
+    @staticmethod
+    def anneal_one_step(cobj, G, beta, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_anneal_one_step(cobj, float(G), float(beta)))
+
+    @staticmethod
+    def get_counters(cobj, dtype):
+        c = (ctypes.c_ulonglong * 3)()
+        _check(_lib.sqaod_rocm_dg_annealer_get_counters(cobj, c))
+        return {'flips': c[0], 'batches': c[1], 'blocked': c[2]}
+
+    @staticmethod
+    def get_energy_rounding_error(cobj, dtype):
+        e = _f64()
+        _check(_lib.sqaod_rocm_dg_annealer_get_energy_rounding_error(cobj, ctypes.byref(e)))
+        return e.value
+
+
+rocm_device = _Device
+rocm_dg_annealer = _DenseGraphAnnealer

@@ -1,0 +1,28 @@
+"""rocm dense graph annealer (sqaodpy/sqaod/cuda/dense_graph_annealer.py:1-34)."""
+import numpy as np
+
+from ..common.dense_graph_annealer_base import DenseGraphAnnealerBase
+from ..common.preference import minimize
+from ._cext import rocm_dg_annealer as cext
+from . import device
+
+
+class DenseGraphAnnealer(DenseGraphAnnealerBase):
+    """Dense graph annealer on the rocm device."""
+
+    def __init__(self, W, optimize, dtype, prefdict):
+        self._cobj = cext.new(dtype)
+        dev = device.get_active_device()
+        cext.assign_device(self._cobj, dev._cobj, dtype)
+        self._device = dev
+        DenseGraphAnnealerBase.__init__(self, cext, dtype, W, optimize, prefdict)
+
+    def get_counters(self):
+        """Accepted flips, committed batches and blocked waits since prepare()."""
+        return cext.get_counters(self._cobj, self.dtype)
+
+
+def dense_graph_annealer(W=None, optimize=minimize, dtype=np.float64, **prefs):
+    """Factory of sqaod_amd.rocm.DenseGraphAnnealer (same signature as
+    sqaod.cuda.dense_graph_annealer)."""
+    return DenseGraphAnnealer(W, optimize, dtype, prefs)

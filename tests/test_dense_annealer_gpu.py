@@ -1,0 +1,213 @@
+"""GPU parity of the rocm dense annealer against the CPU oracle.
+
+Bit-exact cases use integer-valued W (J quarter-integral, every field and
+energy exact in float) and the device MT19937 stream (rng=mt19937), which
+replays the serial CPU annealer's random stream; the oracle is the C
+restatement of CPUDenseGraphAnnealer (oracle/).  Float cases compare energies
+with an FP64 recompute within stated tolerances.
+"""
+import numpy as np
+import pytest
+
+from problems import dense_int_W, dense_normal_W, schedule, SEED_BASE, dense_graph_8x8
+
+pytestmark = pytest.mark.gpu
+
+BETA = 50.0
+
+
+@pytest.fixture(scope="module")
+def rocm():
+    import sqaod_amd.rocm as rocm
+    return rocm
+
+
+def make_pair(oracle, rocm, W, m, seed, algo="coloring", dtype=np.float32, **prefs):
+    N = W.shape[0]
+    ora = oracle.DenseGraphAnnealer(dtype, 1)
+    ora.seed(seed)
+    ora.set_qubo(W.astype(dtype))
+    ora.set_m(m)
+    ora.set_algorithm({"coloring": 3, "sa_naive": 6}[algo])
+    ora.prepare()
+    ora.randomize_spin()
+    an = rocm.dense_graph_annealer(W.astype(dtype), dtype=dtype, n_trotters=m, algorithm=algo,
+                                   rng="mt19937", **prefs)
+    an.seed(seed)
+    an.prepare()
+    an.randomize_spin()
+    return ora, an
+
+
+def q_of(an):
+    return np.stack(an.get_q())
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(field_precision="bf16x2"), dict(sweep="plain"),
+                                     dict(slices_per_block=1)])
+def test_config1_bit_exact_trajectory(oracle, rocm, variant):
+    """N=128, m=32, integer W in [-8, 8], G 5.0 -> 0.01 (x0.97), beta=50: the
+    spins of every slice and the energies equal the serial CPU annealer's
+    after every step of the schedule."""
+    N, m = 128, 32
+    W = dense_int_W(N, SEED_BASE)
+    ora, an = make_pair(oracle, rocm, W, m, seed=1000, **variant)
+    assert np.array_equal(q_of(an), ora.get_q()), "randomize_spin differs"
+    for step, G in enumerate(schedule()):
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+        if step % 10 == 0 or step == len(schedule()) - 1:
+            qa, qo = q_of(an), ora.get_q()
+            assert np.array_equal(qa, qo), f"step {step}: {np.sum(qa != qo)} spins differ"
+    an.make_solution()
+    E = an.get_E()
+    assert np.array_equal(E, ora.calculate_E())
+    x = np.stack(an.get_x())
+    assert np.array_equal(x, (ora.get_q() + 1) // 2)
+
+
+@pytest.mark.parametrize("N,m", [(100, 5), (64, 2), (37, 3), (200, 7), (256, 16)])
+def test_odd_shapes_bit_exact(oracle, rocm, N, m):
+    """odd m (ring neighbours 0 and m-1 share a colour), m = 2, N not a
+    multiple of 32: bit-exact over 30 steps."""
+    W = dense_int_W(N, SEED_BASE + N + m)
+    ora, an = make_pair(oracle, rocm, W, m, seed=7 + m)
+    for G in schedule()[:30]:
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+    assert np.array_equal(q_of(an), ora.get_q())
+
+
+def test_sa_naive_bit_exact(oracle, rocm):
+    """m = 1 selects SA (CPUDenseGraphAnnealer.cpp:174-175); kT = G."""
+    N, m = 96, 1
+    W = dense_int_W(N, SEED_BASE + 5)
+    ora, an = make_pair(oracle, rocm, W, m, seed=3, algo="sa_naive")
+    assert an.get_preferences()["algorithm"] == "sa_naive"
+    for kT in np.geomspace(20.0, 0.05, 60):
+        ora.anneal_one_step(kT, 0.0)
+        an.anneal_one_step(kT, 0.0)
+    assert np.array_equal(q_of(an), ora.get_q())
+
+
+def test_maximize_bit_exact(oracle, rocm):
+    N, m = 64, 8
+    W = dense_int_W(N, SEED_BASE + 9)
+    ora = oracle.DenseGraphAnnealer(np.float32, 1)
+    ora.seed(11)
+    ora.set_qubo(W, oracle.MAXIMIZE)
+    ora.set_m(m)
+    ora.prepare()
+    ora.randomize_spin()
+    import sqaod_amd as sq
+    an = rocm.dense_graph_annealer(W, sq.maximize, dtype=np.float32, n_trotters=m, rng="mt19937")
+    an.seed(11)
+    an.prepare()
+    an.randomize_spin()
+    for G in schedule()[:40]:
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+    assert np.array_equal(q_of(an), ora.get_q())
+    assert np.array_equal(an.get_E(), ora.calculate_E())
+
+
+def test_float_energy_fp64(oracle, rocm):
+    """W ~ N(0,1): get_E is the FP64 recompute rounded to float; it agrees
+    with the oracle's FP64 energy of the same spins within 1e-5 relative."""
+    N, m = 512, 16
+    W = dense_normal_W(N, SEED_BASE + 2)
+    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m)
+    an.seed(5)
+    an.prepare()
+    an.randomize_spin()
+    for G in schedule()[:50]:
+        an.anneal_one_step(G, BETA)
+    E = an.get_E()
+    q = np.stack(an.get_q())
+    h, J, c = oracle.dense_hamiltonian(W.astype(np.float64))
+    Eref = oracle.dense_energy_q(h, J, c, q)
+    assert np.allclose(E, Eref, rtol=1e-5, atol=1e-3)
+
+
+def test_local_vs_plain_float(rocm):
+    """The local-field sweep and the plain per-attempt-dot sweep make the same
+    decisions on float data with the same stream (fields agree to rounding)."""
+    N, m = 256, 8
+    W = dense_normal_W(N, SEED_BASE + 3)
+    res = []
+    for sweep in ("local", "plain"):
+        an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m, sweep=sweep)
+        an.seed(21)
+        an.prepare()
+        an.randomize_spin()
+        for G in schedule()[:20]:
+            an.anneal_one_step(G, BETA)
+        res.append(np.stack(an.get_q()))
+    frac = np.mean(res[0] != res[1])
+    assert frac < 0.05
+
+
+def test_known_answers(rocm):
+    """sqaodpy/tests/test_dense_graph_annealer.py:85-96 and :154-178."""
+    N = 8
+    an = rocm.dense_graph_annealer(np.ones((N, N), np.float32), dtype=np.float32, n_trotters=1)
+    an.prepare()
+    an.set_q(np.ones(N, np.int8))
+    E = an.get_E()
+    assert len(E) == 1 and E[0] == N * N
+    x = an.get_x()
+    assert len(x) == 1 and np.all(x[0] == 1)
+    an.set_q(-np.ones(N, np.int8))
+    assert an.get_E()[0] == 0.
+
+
+def test_trivial_problem_reaches_zero(rocm):
+    """test_dense_graph_annealer.py:180-260: W = ones(10); minimize reaches E=0."""
+    import sqaod_amd as sq
+    for opt, sign in ((sq.minimize, 1), (sq.maximize, -1)):
+        W = sign * np.ones((10, 10), np.float64)
+        an = rocm.dense_graph_annealer(W, opt, dtype=np.float64, n_trotters=4)
+        an.seed(0)
+        an.prepare()
+        an.randomize_spin()
+        G = 5.0
+        while G > 0.02:
+            an.anneal_one_step(G, 1. / 0.03)
+            G *= 0.95
+        an.make_solution()
+        E = an.get_E()
+        assert np.min(E) == 0 if opt is sq.minimize else np.max(E) == 0
+
+
+def test_8x8_example(rocm, oracle):
+    W = dense_graph_8x8(np.float32)
+    Emin, _ = oracle.dense_bf_search(W.astype(np.float64))
+    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=4)
+    an.seed(1)
+    an.prepare()
+    an.randomize_spin()
+    for G in schedule():
+        an.anneal_one_step(G, BETA)
+    assert np.min(an.get_E()) == Emin
+
+
+def test_large_instance_runs(rocm, oracle):
+    """N=2048, m=64, W ~ N(0,1): the sweep completes (no neighbour-wait
+    abort), the energy decreases, and E agrees with an FP64 recompute."""
+    N, m = 2048, 64
+    W = dense_normal_W(N, SEED_BASE + 4)
+    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m)
+    an.seed(3)
+    an.prepare()
+    an.randomize_spin()
+    E0 = np.mean(an.get_E())
+    for G in schedule()[:40]:
+        an.anneal_one_step(G, BETA)
+    E1 = an.get_E()
+    assert np.mean(E1) < E0
+    q = np.stack(an.get_q())
+    h, J, c = oracle.dense_hamiltonian(W.astype(np.float64))
+    Eref = oracle.dense_energy_q(h, J, c, q)
+    assert np.allclose(E1, Eref, rtol=1e-5, atol=1e-2)
+    cnt = an.get_counters()
+    assert cnt["flips"] > 0 and cnt["batches"] > 0
