@@ -105,6 +105,10 @@ int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, 
  * max |E_dtype - E_fp64| of the last energy evaluation. */
 int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]);
 int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err);
+/* With preference "profile" = "1": device time (ms, HIP events on the solver's
+ * stream) and launch counts of [field refresh, random-stream generation, sweep]
+ * accumulated since the previous call; resets the accumulators. */
+int sqaod_rocm_dg_annealer_get_phase_times(sqaod_rocm_dg_annealer *a, double ms[3], unsigned long long counts[3]);
 
 /* Self check without a reference counterpart: order-independent checksum of
  * the device exp (the acceptance test's expf) over float bit patterns

@@ -7,7 +7,7 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/sqaod_rocm.h"
+#include "sqaod_rocm.h"
 #include "solver.h"
 
 using namespace sqaod_rocm;
@@ -297,6 +297,10 @@ int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, 
 
 int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]) {
     return guarded([&] { withDg(a, [&](auto &s) { s.getCounters(counters); }); });
+}
+
+int sqaod_rocm_dg_annealer_get_phase_times(sqaod_rocm_dg_annealer *a, double ms[3], unsigned long long counts[3]) {
+    return guarded([&] { withDg(a, [&](auto &s) { s.getPhaseTimes(ms, counts); }); });
 }
 
 int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err) {

@@ -144,6 +144,65 @@ DenseGraphAnnealer<real>::~DenseGraphAnnealer() {
     }
     if (abortEvent_)
         (void)hipEventDestroy(abortEvent_);
+    for (auto &pe : pending_) {
+        (void)hipEventDestroy(pe.start);
+        (void)hipEventDestroy(pe.stop);
+    }
+    for (auto e : eventPool_)
+        (void)hipEventDestroy(e);
+}
+
+template <class real>
+hipEvent_t DenseGraphAnnealer<real>::takeEvent() {
+    if (!eventPool_.empty()) {
+        hipEvent_t e = eventPool_.back();
+        eventPool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::phaseBegin(int phase, PhaseEvents &pe) {
+    pe.phase = phase;
+    pe.start = takeEvent();
+    pe.stop = takeEvent();
+    HIP_CHECK(hipEventRecord(pe.start, dev_->stream()));
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::phaseEnd(PhaseEvents &pe) {
+    HIP_CHECK(hipEventRecord(pe.stop, dev_->stream()));
+    pending_.push_back(pe);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::collectPhases() {
+    for (auto &pe : pending_) {
+        HIP_CHECK(hipEventSynchronize(pe.stop));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, pe.start, pe.stop));
+        phaseMs_[pe.phase] += ms;
+        phaseCount_[pe.phase] += 1;
+        eventPool_.push_back(pe.start);
+        eventPool_.push_back(pe.stop);
+    }
+    pending_.clear();
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::getPhaseTimes(double ms[3], unsigned long long counts[3]) {
+    if (dev_)
+        dev_->makeCurrent();
+    collectPhases();
+    for (int i = 0; i < 3; ++i) {
+        ms[i] = phaseMs_[i];
+        counts[i] = phaseCount_[i];
+        phaseMs_[i] = 0.;
+        phaseCount_[i] = 0;
+    }
 }
 
 template <class real>
@@ -247,13 +306,17 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
                 ensureDevice();
                 d_Jhi_.allocate((size_t)N_ * Np_);
                 d_Jlo_.allocate((size_t)N_ * Np_);
-                splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, dev_->stream());
+                d_Jsplit_.allocate((size_t)N_ * Np_);
+                splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, d_Jsplit_.ptr,
+                          dev_->stream());
             }
         }
     } else if (name == "slices_per_block") {
         slicesPerBlock_ = atoi(value.c_str());
     } else if (name == "refresh_interval") {
         refreshInterval_ = atoi(value.c_str());
+    } else if (name == "profile") {
+        profile_ = (value == "1" || value == "on" || value == "true" || value == "True");
     } else if (name == "spin_limit") {
         spinLimit_ = atoi(value.c_str());
     } else if (name == "precision" || name == "device") {
@@ -278,6 +341,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("field_precision", fieldFormat_ == fieldBF16x2 ? "bf16x2" : (sizeof(real) == 4 ? "fp32" : "fp64"));
     p.emplace_back("slices_per_block", std::to_string(slicesPerBlock_));
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
+    p.emplace_back("profile", profile_ ? "1" : "0");
     return p;
 }
 
@@ -302,7 +366,8 @@ void DenseGraphAnnealer<real>::uploadJ(const real *J, const real *h, real c) {
     if (fieldFormat_ == fieldBF16x2) {
         d_Jhi_.allocate((size_t)N_ * Np_);
         d_Jlo_.allocate((size_t)N_ * Np_);
-        splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, dev_->stream());
+        d_Jsplit_.allocate((size_t)N_ * Np_);
+        splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, d_Jsplit_.ptr, dev_->stream());
     }
     c_ = c;
     h_.assign(h, h + N_);
@@ -398,9 +463,9 @@ void DenseGraphAnnealer<real>::prepare() {
     d_partial_.allocate((size_t)m_ * (Np_ / 256));
     d_spinDot_.allocate(1);
     d_abort_.allocate(1);
-    d_counters_.allocate(3);
+    d_counters_.allocate(16); /* [0..2] public, [3..8] optional per-phase cycles */
     HIP_CHECK(hipMemsetAsync(d_abort_.ptr, 0, sizeof(int), dev_->stream()));
-    HIP_CHECK(hipMemsetAsync(d_counters_.ptr, 0, 3 * sizeof(unsigned long long), dev_->stream()));
+    HIP_CHECK(hipMemsetAsync(d_counters_.ptr, 0, 16 * sizeof(unsigned long long), dev_->stream()));
     h_abort_.allocate(1);
     h_abort_.ptr[0] = 0;
     if (!abortEvent_)
@@ -518,9 +583,19 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     checkAbort(false);
     if (!fieldsDirty_ && refreshInterval_ > 0 && stepsSinceRefresh_ >= refreshInterval_)
         fieldsDirty_ = true;
-    if (fieldsDirty_)
+    PhaseEvents pe;
+    if (fieldsDirty_) {
+        if (profile_)
+            phaseBegin(0, pe);
         refreshFields();
+        if (profile_)
+            phaseEnd(pe);
+    }
+    if (profile_)
+        phaseBegin(1, pe);
     generateAttempts();
+    if (profile_)
+        phaseEnd(pe);
 
     SweepArgs<real> a{};
     a.N = N_;
@@ -533,9 +608,7 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     a.coef = std::log(std::tanh(G * beta / real(m_))) / beta;
     a.beta = beta;
     a.invKT = real(1.) / G;
-    a.J = d_J_.ptr;
-    a.Jhi = d_Jhi_.ptr;
-    a.Jlo = d_Jlo_.ptr;
+    a.J = (fieldFormat_ == fieldBF16x2 && !plainSweep_) ? reinterpret_cast<const real *>(d_Jsplit_.ptr) : d_J_.ptr;
     a.h = d_h_.ptr;
     a.F = d_F_.ptr;
     a.Qbits = d_Q_.ptr;
@@ -554,14 +627,20 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
 
     int S = slicesPerBlock_;
     if (S <= 0) {
-        S = 8;
+        S = 4;
         while (S > 1 && (denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_) > dev_->maxLdsPerBlock() || S > m_))
             S >>= 1;
     }
     SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_) > dev_->maxLdsPerBlock(),
                 "N=%d too large for the LDS-resident sweep (%zu bytes needed per block)", N_,
                 denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_));
+    if (profile_)
+        phaseBegin(2, pe);
     denseSweep<real>(a, plainSweep_, fieldFormat_, S, dev_->stream());
+    if (profile_)
+        phaseEnd(pe);
+    if (pending_.size() > 3000)
+        collectPhases();
 
     HIP_CHECK(hipMemcpyAsync(h_abort_.ptr, d_abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, dev_->stream()));
     HIP_CHECK(hipEventRecord(abortEvent_, dev_->stream()));

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kCols) void fieldRefreshKernel(const real *J, const
             F[(long long)(y0 + t) * Np + i] = acc[t];
 }
 
-__global__ void splitBF16Kernel(const float *J, size_t count, uint16_t *hi, uint16_t *lo) {
+__global__ void splitBF16Kernel(const float *J, size_t count, uint16_t *hi, uint16_t *lo, float *Jsplit) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= count)
         return;
@@ -73,14 +73,15 @@ __global__ void splitBF16Kernel(const float *J, size_t count, uint16_t *hi, uint
     uint32_t rl = bl + 0x7fffu + ((bl >> 16) & 1u);
     hi[idx] = h;
     lo[idx] = (uint16_t)(rl >> 16);
+    Jsplit[idx] = hv + __uint_as_float((rl >> 16) << 16);
 }
 
 }  // namespace
 
-void splitBF16(const float *d_J, size_t count, uint16_t *d_hi, uint16_t *d_lo, hipStream_t stream) {
+void splitBF16(const float *d_J, size_t count, uint16_t *d_hi, uint16_t *d_lo, float *d_Jsplit, hipStream_t stream) {
     if (!count)
         return;
-    splitBF16Kernel<<<(unsigned)((count + 255) / 256), 256, 0, stream>>>(d_J, count, d_hi, d_lo);
+    splitBF16Kernel<<<(unsigned)((count + 255) / 256), 256, 0, stream>>>(d_J, count, d_hi, d_lo, d_Jsplit);
     HIP_CHECK(hipGetLastError());
 }
 

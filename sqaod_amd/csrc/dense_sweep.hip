@@ -38,18 +38,22 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kTblMax = 4096; /* entries of the pending-site table per slice */
+constexpr int kDotGroup = 8; /* plain sweep: attempts whose dots are formed together */
+constexpr int kMaxSlicesPerBlock = 4; /* one wave per SIMD, so a wave may hold up to 256 VGPRs */
+constexpr size_t kExpTabBytes = 32 * sizeof(uint64_t); /* LDS copy of the exp table */
 
-template <class real>
-struct Ops;
-template <>
-struct Ops<float> {
-    /* the CPU annealer's expf(-dE*beta), bit-identical (sq_expf.h) */
-    __device__ static float expo(float v) { return sq_expf(v); }
-};
-template <>
-struct Ops<double> {
-    __device__ static double expo(double v) { return exp(v); }
-};
+/* Optional per-phase cycle counters (build with -DSQAOD_SWEEP_CYCLES): shader
+ * clock cycles spent per phase, summed into SweepArgs::counters[3..10]. */
+#ifdef SQAOD_SWEEP_CYCLES
+#define SQ_CYC(v) const unsigned long long v = __builtin_amdgcn_s_memtime()
+#define SQ_CYC_ADD(acc, a, b) (acc) += (b) - (a)
+#define SQ_CYC_WAITALL() __builtin_amdgcn_s_waitcnt(0)
+#else
+#define SQ_CYC(v)
+#define SQ_CYC_ADD(acc, a, b)
+#define SQ_CYC_WAITALL()
+#endif
+
 
 __device__ __forceinline__ uint32_t loadShared32(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -82,38 +86,56 @@ __device__ __forceinline__ int slicePos(int y, int m) {
     return (y & 1) ? nEven + (y >> 1) : (y >> 1);
 }
 
-/* Row of J in the sweep's field format, 4 consecutive elements. */
-template <class real, int FMT>
-struct JRow;
+/* 4 consecutive elements of a J row (16- or 32-byte vector load). */
 template <class real>
-struct JRow<real, fieldF32> {
-    __device__ static void load4(const SweepArgs<real> &a, long long off, real v[4]) {
-        const real *p = a.J + off;
-        if constexpr (sizeof(real) == 4) {
-            float4 t = *reinterpret_cast<const float4 *>(p);
-            v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-        } else {
-            double2 t0 = *reinterpret_cast<const double2 *>(p);
-            double2 t1 = *reinterpret_cast<const double2 *>(p + 2);
-            v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
-        }
+__device__ __forceinline__ void loadJ4(const real *p, real v[4]) {
+    if constexpr (sizeof(real) == 4) {
+        float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+        double2 t0 = *reinterpret_cast<const double2 *>(p);
+        double2 t1 = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
     }
+}
+
+template <class real>
+__device__ __forceinline__ void ldsLoad4(const real *p, real v[4]) {
+    if constexpr (sizeof(real) == 4) {
+        float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+        double2 t0 = *reinterpret_cast<const double2 *>(p);
+        double2 t1 = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
+    }
+}
+
+template <class real>
+__device__ __forceinline__ void ldsStore4(real *p, const real v[4]) {
+    if constexpr (sizeof(real) == 4) {
+        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(p + 2) = make_double2(v[2], v[3]);
+    }
+}
+
+template <class real>
+struct Expo;
+template <>
+struct Expo<float> {
+    /* the CPU annealer's expf(-dE*beta), bit-identical (sq_expf.h) */
+    __device__ static float eval(float v, const uint64_t *tab) { return sq_expf_tab(v, tab); }
 };
-template <class real>
-struct JRow<real, fieldBF16x2> {
-    __device__ static void load4(const SweepArgs<real> &a, long long off, real v[4]) {
-        uint2 hi = *reinterpret_cast<const uint2 *>(a.Jhi + off);
-        uint2 lo = *reinterpret_cast<const uint2 *>(a.Jlo + off);
-        uint32_t h[4] = {hi.x << 16, hi.x & 0xffff0000u, hi.y << 16, hi.y & 0xffff0000u};
-        uint32_t l[4] = {lo.x << 16, lo.x & 0xffff0000u, lo.y << 16, lo.y & 0xffff0000u};
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            v[t] = (real)(__uint_as_float(h[t]) + __uint_as_float(l[t])); /* exact: 16 significant bits */
-    }
+template <>
+struct Expo<double> {
+    __device__ static double eval(double v, const uint64_t *) { return exp(v); }
 };
 
-template <class real, bool PLAIN, int FMT>
-__global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdRemap) {
+template <class real, bool PLAIN>
+__global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
+                                                                                int xcdRemap) {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char lds[];
     const int N = a.N, Np = a.Np, m = a.m, Nw = a.Nw;
@@ -131,9 +153,10 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
     }
     const int y = b * slicesPerBlock + w;
 
-    /* LDS carve-up */
-    real *hS = reinterpret_cast<real *>(lds);
-    unsigned char *p = lds + (size_t)Np * sizeof(real);
+    /* LDS carve-up: exp table, h, then per wave F, pending table, spin bits, mirrors */
+    uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
+    real *hS = reinterpret_cast<real *>(lds + kExpTabBytes);
+    unsigned char *p = lds + kExpTabBytes + (size_t)Np * sizeof(real);
     const size_t perWave = (PLAIN ? 0 : (size_t)Np * sizeof(real)) + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
     p += perWave * w;
     real *F = reinterpret_cast<real *>(p);
@@ -141,6 +164,8 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
     uint32_t *Qb = reinterpret_cast<uint32_t *>(tbl + tblN);
     uint32_t *Mir[2] = {Qb + Nw, Qb + 2 * Nw};
 
+    if (threadIdx.x < 32)
+        expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
     for (int i = threadIdx.x; i < Np; i += blockDim.x)
         hS[i] = a.h[i];
     __syncthreads();
@@ -173,57 +198,111 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
     int spins = 0;
     bool aborted = false;
 
+    /* Own attempt stream: a rolling window of 128 entries in two halves
+     * (entries wk + lane and wk + 64 + lane); a batch at k < wk + 64 reads
+     * lanes k - wk .. k - wk + 63 of it, and a half is refilled as soon as it
+     * is consumed, so the stream is rarely on the critical path. */
+    uint32_t wS0, wS1;
+    real wU0, wU1;
+    int wk = 0;
+    auto loadHalf = [&](int base, uint32_t &sv, real &uv) {
+        const int kk = base + lane;
+        sv = kk < N ? a.X[yN + kk] : 0u;
+        uv = kk < N ? a.U[yN + kk] : (real)0;
+    };
+    loadHalf(0, wS0, wU0);
+    loadHalf(kWave, wS1, wU1);
+
+    /* Neighbour windows: decisions and sites of entries [applied, applied+128),
+     * fetched one batch ahead. */
+    uint32_t nd[2][2], ns[2][2];
+    int nbase[2] = {0, 0};
+    auto loadNeighbour = [&](int nb) {
+        const long long nN = (long long)nbr[nb] * N;
+        const int base = applied[nb];
+        const int e0 = base + lane, e1 = base + kWave + lane;
+        nbase[nb] = base;
+        nd[nb][0] = e0 < N ? loadShared32(a.D + nN + e0) : 0u;
+        nd[nb][1] = e1 < N ? loadShared32(a.D + nN + e1) : 0u;
+        ns[nb][0] = e0 < N ? a.X[nN + e0] : 0u;
+        ns[nb][1] = e1 < N ? a.X[nN + e1] : 0u;
+    };
+    if (!a.sa) {
+        loadNeighbour(0);
+        loadNeighbour(1);
+    }
+
+    unsigned long long cyc[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* load wait, neighbours, evaluate, commit, row, blocked */
+    (void)cyc;
     while (k < N) {
-        /* own attempts k + lane */
-        const int kk = k + lane;
-        const bool inRange = kk < N;
-        const uint32_t s = inRange ? a.X[yN + kk] : 0u;
-        const real u = inRange ? a.U[yN + kk] : (real)0;
+        SQ_CYC(c0);
+        SQ_CYC_WAITALL();
+        SQ_CYC(c1);
+        SQ_CYC_ADD(cyc[0], c0, c1);
+        if (k >= wk + kWave) {
+            wS0 = wS1;
+            wU0 = wU1;
+            wk += kWave;
+            loadHalf(wk + kWave, wS1, wU1);
+        }
         int limit = N - k;
+        uint32_t s;
+        real u;
+        {
+            const int idx = k - wk + lane; /* < 128 */
+            const uint32_t sa0 = __shfl(wS0, idx & (kWave - 1)), sa1 = __shfl(wS1, idx & (kWave - 1));
+            const real ua0 = __shfl(wU0, idx & (kWave - 1)), ua1 = __shfl(wU1, idx & (kWave - 1));
+            s = idx < kWave ? sa0 : sa1;
+            u = idx < kWave ? ua0 : ua1;
+            if (k + lane >= N)
+                s = 0u;
+        }
 
         /* neighbours: replay decided prefixes, register pending entries */
-        uint32_t pendSite[2][2];
-        bool pendSet[2][2];
+        bool pendSet[2][2] = {{false, false}, {false, false}};
         if (!a.sa) {
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
                 const long long nN = (long long)nbr[nb] * N;
                 const int Kk = k + delta[nb];
-                int base;
-                uint32_t d0, d1, s0, s1;
-                int e0, e1;
                 for (;;) {
-                    base = applied[nb];
-                    e0 = base + lane;
-                    e1 = base + kWave + lane;
-                    d0 = e0 < N ? loadShared32(a.D + nN + e0) : 0u;
-                    d1 = e1 < N ? loadShared32(a.D + nN + e1) : 0u;
-                    s0 = e0 < N ? a.X[nN + e0] : 0u;
-                    s1 = e1 < N ? a.X[nN + e1] : 0u;
-                    const bool dec0 = (d0 >> 1) == tag, dec1 = (d1 >> 1) == tag;
+                    const int base = nbase[nb];
+                    const int e0 = base + lane, e1 = base + kWave + lane;
+                    const bool dec0 = (nd[nb][0] >> 1) == tag, dec1 = (nd[nb][1] >> 1) == tag;
                     const unsigned long long u0 = __ballot(e0 < N && !dec0);
                     const unsigned long long u1 = __ballot(e1 < N && !dec1);
-                    int winLen = N - base < 2 * kWave ? N - base : 2 * kWave;
-                    int pfx = u0 ? firstLane(u0) : (u1 ? kWave + firstLane(u1) : winLen);
-                    int applyEnd = base + pfx < Kk ? base + pfx : Kk;
-                    if (e0 < applyEnd && (d0 & 1u))
-                        atomicXor(&Mir[nb][s0 >> 5], 1u << (s0 & 31));
-                    if (e1 < applyEnd && (d1 & 1u))
-                        atomicXor(&Mir[nb][s1 >> 5], 1u << (s1 & 31));
-                    applied[nb] = applyEnd;
-                    if (pfx == 2 * kWave && applyEnd == base + 2 * kWave && applyEnd < Kk)
-                        continue; /* whole window replayed, still behind: next window */
+                    const int winLen = N - base < 2 * kWave ? N - base : 2 * kWave;
+                    const int pfx = u0 ? firstLane(u0) : (u1 ? kWave + firstLane(u1) : winLen);
+                    const int from = applied[nb];
+                    const int applyEnd = base + pfx < Kk ? base + pfx : Kk;
+                    const bool x0 = e0 >= from && e0 < applyEnd && (nd[nb][0] & 1u);
+                    const bool x1 = e1 >= from && e1 < applyEnd && (nd[nb][1] & 1u);
+                    if (__ballot(x0 || x1)) {
+                        if (x0)
+                            atomicXor(&Mir[nb][ns[nb][0] >> 5], 1u << (ns[nb][0] & 31));
+                        if (x1)
+                            atomicXor(&Mir[nb][ns[nb][1] >> 5], 1u << (ns[nb][1] & 31));
+                    }
+                    if (applyEnd > applied[nb])
+                        applied[nb] = applyEnd;
+                    if (pfx == 2 * kWave && applyEnd == base + 2 * kWave && applyEnd < Kk) {
+                        /* whole window replayed and still behind: fetch the next one */
+                        loadNeighbour(nb);
+                        (void)nN;
+                        continue;
+                    }
                     break;
                 }
                 const int A = applied[nb];
-                pendSet[nb][0] = e0 >= A && e0 < N;
-                pendSet[nb][1] = e1 >= A && e1 < N;
-                pendSite[nb][0] = s0;
-                pendSite[nb][1] = s1;
+                const int base = nbase[nb];
+                const int e0 = base + lane, e1 = base + kWave + lane;
+                /* an entry blocks lane i only if e - Kk < i <= 63 */
+                pendSet[nb][0] = e0 >= A && e0 < N && e0 - Kk < kWave - 1;
+                pendSet[nb][1] = e1 >= A && e1 < N && e1 - Kk < kWave - 1;
                 if (pendSet[nb][0])
-                    atomicMin(&tbl[s0 & tblMask], e0 - Kk);
+                    atomicMin(&tbl[ns[nb][0] & tblMask], e0 - Kk);
                 if (pendSet[nb][1])
-                    atomicMin(&tbl[s1 & tblMask], e1 - Kk);
+                    atomicMin(&tbl[ns[nb][1] & tblMask], e1 - Kk);
                 /* lane i needs entries [A, Kk + i): they must lie in the window */
                 const int winEnd = base + 2 * kWave < N ? base + 2 * kWave : N;
                 if (winEnd < N) {
@@ -233,6 +312,8 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
             }
         }
         waveSync();
+        SQ_CYC(c2);
+        SQ_CYC_ADD(cyc[1], c1, c2);
 
         /* evaluate attempt k + lane speculatively */
         const bool valid = lane < limit;
@@ -241,73 +322,133 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
             blocked = tbl[s & tblMask] < lane;
         const uint32_t qword = Qb[s >> 5];
         const bool qpos = (qword >> (s & 31)) & 1u;
-        real f;
-        if constexpr (PLAIN) {
-            /* reference algorithm: field = J[s] . q_y, recomputed per attempt */
-            real acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-            if (valid && !blocked) {
-                const long long rowOff = (long long)s * Np;
-                for (int j = 0; j < Np; j += 32) {
-                    const uint32_t qw = Qb[j >> 5];
-#pragma unroll
-                    for (int t = 0; t < 32; t += 4) {
-                        real v[4];
-                        JRow<real, fieldF32>::load4(a, rowOff + j + t, v);
-                        acc0 += ((qw >> (t + 0)) & 1u) ? v[0] : -v[0];
-                        acc1 += ((qw >> (t + 1)) & 1u) ? v[1] : -v[1];
-                        acc2 += ((qw >> (t + 2)) & 1u) ? v[2] : -v[2];
-                        acc3 += ((qw >> (t + 3)) & 1u) ? v[3] : -v[3];
-                    }
-                }
-            }
-            f = (acc0 + acc1) + (acc2 + acc3);
-        } else {
-            f = F[s];
-        }
         const real q = qpos ? (real)1 : (real)-1;
         const real hx = hS[s];
-        real thr;
+        real nsum = 0;
         if (!a.sa) {
             const real nv0 = ((Mir[0][s >> 5] >> (s & 31)) & 1u) ? (real)1 : (real)-1;
             const real nv1 = ((Mir[1][s >> 5] >> (s & 31)) & 1u) ? (real)1 : (real)-1;
-            /* CPUDenseGraphAnnealer.cpp:258-262, same operation order */
-            real dE = a.twoDivM * q * (hx + (real)2. * f);
-            dE -= q * (nv0 + nv1) * a.coef;
-            thr = (dE < (real)0.) ? (real)1. : Ops<real>::expo(-dE * a.beta);
-        } else {
-            /* :348-349 */
-            real dE = (real)2. * q * (hx + (real)2. * f);
-            thr = (dE < (real)0.) ? (real)1. : Ops<real>::expo(-dE * a.invKT);
+            nsum = nv0 + nv1;
         }
-        const bool acc = thr > u;
+        /* the pending entries of this batch are now in the table; the next
+         * batch's neighbour windows can be fetched while this one evaluates */
+        bool pendSetNow[2][2] = {{pendSet[0][0], pendSet[0][1]}, {pendSet[1][0], pendSet[1][1]}};
+        uint32_t pendSite[2][2] = {{ns[0][0], ns[0][1]}, {ns[1][0], ns[1][1]}};
+        auto decide = [&](real f) {
+            real thr;
+            if (!a.sa) {
+                /* CPUDenseGraphAnnealer.cpp:258-262, same operation order */
+                real dE = a.twoDivM * q * (hx + (real)2. * f);
+                dE -= q * nsum * a.coef;
+                thr = (dE < (real)0.) ? (real)1. : Expo<real>::eval(-dE * a.beta, expTab);
+            } else {
+                /* :348-349 */
+                real dE = (real)2. * q * (hx + (real)2. * f);
+                thr = (dE < (real)0.) ? (real)1. : Expo<real>::eval(-dE * a.invKT, expTab);
+            }
+            return thr > u;
+        };
         const unsigned long long stopMask = __ballot(!valid || blocked);
-        const unsigned long long accMask = __ballot(valid && !blocked && acc);
         const int firstStop = firstLane(stopMask);
-        const int firstAcc = firstLane(accMask);
-        int c, flipLane;
-        if (firstAcc < firstStop) {
-            c = firstAcc + 1;
-            flipLane = firstAcc;
+        int c = firstStop, flipLane = -1;
+        if constexpr (PLAIN) {
+            /* reference algorithm: field = J[s] . q_y recomputed per attempt,
+             * as coalesced wave-wide dots, eight attempts at a time, stopping
+             * at the first accepted flip */
+            real f = 0;
+            for (int g = 0; g < firstStop && flipLane < 0; g += kDotGroup) {
+                real acc[kDotGroup];
+                uint32_t site[kDotGroup];
+#pragma unroll
+                for (int t = 0; t < kDotGroup; ++t) {
+                    acc[t] = 0;
+                    site[t] = __shfl(s, (g + t) & (kWave - 1));
+                }
+                const int nDots = firstStop - g < kDotGroup ? firstStop - g : kDotGroup;
+                for (int j = lane * 4; j < Np; j += kWave * 4) {
+                    const uint32_t qw = Qb[j >> 5] >> (j & 31);
+#pragma unroll
+                    for (int t = 0; t < kDotGroup; ++t) {
+                        if (t < nDots) {
+                            real v[4];
+                            loadJ4(a.J + (long long)site[t] * Np + j, v);
+                            acc[t] += ((qw & 1u) ? v[0] : -v[0]) + ((qw & 2u) ? v[1] : -v[1]) +
+                                      ((qw & 4u) ? v[2] : -v[2]) + ((qw & 8u) ? v[3] : -v[3]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < kDotGroup; ++t) {
+                    real v = acc[t];
+                    for (int off = kWave / 2; off > 0; off >>= 1)
+                        v += __shfl_xor(v, off);
+                    if (lane == g + t)
+                        f = v;
+                }
+                const bool inGroup = lane >= g && lane < g + nDots;
+                const unsigned long long accMask = __ballot(inGroup && decide(f));
+                if (accMask)
+                    flipLane = firstLane(accMask);
+            }
+            if (flipLane >= 0)
+                c = flipLane + 1;
         } else {
-            c = firstStop;
-            flipLane = -1;
+            const bool acc = decide(F[s]);
+            const unsigned long long accMask = __ballot(valid && !blocked && acc);
+            const int firstAcc = firstLane(accMask);
+            if (firstAcc < firstStop) {
+                c = firstAcc + 1;
+                flipLane = firstAcc;
+            }
         }
         waveSync();
+        SQ_CYC(c3);
+        SQ_CYC_ADD(cyc[2], c2, c3);
 
         /* clear the pending-site table */
         if (!a.sa) {
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
-                if (pendSet[nb][0])
+                if (pendSetNow[nb][0])
                     tbl[pendSite[nb][0] & tblMask] = 0x7fffffff;
-                if (pendSet[nb][1])
+                if (pendSetNow[nb][1])
                     tbl[pendSite[nb][1] & tblMask] = 0x7fffffff;
             }
         }
 
         if (c == 0) {
-            /* lane 0 waits on a neighbour's serially earlier attempt */
+            /* lane 0 waits on a neighbour's serially earlier attempt: poll the
+             * earliest pending entry at lane 0's site (or, when the window does
+             * not reach lane 0's attempt, the first undecided entry) */
             ++nBlocked;
+            if (!a.sa) {
+                const uint32_t s0 = __shfl(s, 0);
+                int waitNb = -1, waitE = 0x7fffffff;
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    const int base = nbase[nb];
+                    const int Kk = k + delta[nb];
+                    const int e0 = base + lane, e1 = base + kWave + lane;
+                    const bool m0 = pendSetNow[nb][0] && pendSite[nb][0] == s0 && e0 < Kk;
+                    const bool m1 = pendSetNow[nb][1] && pendSite[nb][1] == s0 && e1 < Kk;
+                    const unsigned long long b0 = __ballot(m0), b1 = __ballot(m1);
+                    int e = b0 ? base + firstLane(b0) : (b1 ? base + kWave + firstLane(b1) : 0x7fffffff);
+                    if (e == 0x7fffffff && limit <= 0 && applied[nb] < Kk)
+                        e = applied[nb]; /* window short of Kk: the first undecided entry */
+                    if (e < waitE) {
+                        waitE = e;
+                        waitNb = nb;
+                    }
+                }
+                if (waitNb >= 0 && waitE < N) {
+                    const uint32_t *dp = a.D + (long long)nbr[waitNb] * N + waitE;
+                    while ((loadShared32(dp) >> 1) != tag) {
+                        if (++spins > a.spinLimit)
+                            break;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+            }
             if (++spins > a.spinLimit) {
                 atomicExch(a.abortFlag, 1);
                 aborted = true;
@@ -317,8 +458,15 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
                 aborted = true;
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            if (!a.sa) {
+                loadNeighbour(0);
+                loadNeighbour(1);
+            }
             waveSync();
+            {
+                SQ_CYC(cb);
+                SQ_CYC_ADD(cyc[5], c0, cb);
+            }
             continue;
         }
         spins = 0;
@@ -328,6 +476,7 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
         if (lane < c)
             storeShared32(a.D + yN + k + lane, (tag << 1) | (lane == flipLane ? 1u : 0u));
 
+        SQ_CYC(c4);
         if (flipLane >= 0) {
             const uint32_t x = __shfl(s, flipLane);
             const bool qx = __shfl((int)qpos, flipLane) != 0;
@@ -335,20 +484,43 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
             if (lane == 0)
                 atomicXor(&Qb[x >> 5], 1u << (x & 31));
             if constexpr (!PLAIN) {
-                /* F[y][:] += -2 q_x J[:][x] = -2 q_x J[x][:]  (J symmetric) */
+                /* F[y][:] += -2 q_x J[:][x] = -2 q_x J[x][:]  (J symmetric);
+                 * the row's loads go out together, then the LDS updates */
                 const real d = qx ? (real)-2. : (real)2.;
-                const long long rowOff = (long long)x * Np;
-                for (int j = lane * 4; j < Np; j += kWave * 4) {
-                    real v[4];
-                    JRow<real, FMT>::load4(a, rowOff + j, v);
+                const real *row = a.J + (long long)x * Np;
+                constexpr int kRowChunk = sizeof(real) == 4 ? 16 : 8;
+                for (int j0 = lane * 4; j0 < Np; j0 += kWave * 4 * kRowChunk) {
+                    real v[kRowChunk][4];
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        F[j + t] = F[j + t] + d * v[t];
+                    for (int c4 = 0; c4 < kRowChunk; ++c4) {
+                        const int j = j0 + c4 * kWave * 4;
+                        if (j < Np)
+                            loadJ4(row + j, v[c4]);
+                    }
+#pragma unroll
+                    for (int c4 = 0; c4 < kRowChunk; ++c4) {
+                        const int j = j0 + c4 * kWave * 4;
+                        if (j < Np) {
+                            real fv[4];
+                            ldsLoad4(F + j, fv);
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                fv[t] = fv[t] + d * v[c4][t];
+                            ldsStore4(F + j, fv);
+                        }
+                    }
                 }
             }
         }
-        waveSync();
         k += c;
+        if (!a.sa && k < N) {
+            loadNeighbour(0);
+            loadNeighbour(1);
+        }
+        waveSync();
+        SQ_CYC(c5);
+        SQ_CYC_ADD(cyc[3], c3, c5);
+        SQ_CYC_ADD(cyc[4], c4, c5);
     }
 
     if (!aborted) {
@@ -362,6 +534,10 @@ __global__ void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock, int xcdR
         atomicAdd(&a.counters[0], nFlips);
         atomicAdd(&a.counters[1], nBatches);
         atomicAdd(&a.counters[2], nBlocked);
+#ifdef SQAOD_SWEEP_CYCLES
+        for (int t = 0; t < 6; ++t)
+            atomicAdd(&a.counters[3 + t], cyc[t]);
+#endif
     }
 }
 
@@ -371,25 +547,20 @@ size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plai
     int tblN = pendingTableSize(Np);
     int Nw = Np / 32;
     size_t perWave = (plain ? 0 : (size_t)Np * realSize) + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
-    return (size_t)Np * realSize + perWave * slicesPerBlock;
+    return kExpTabBytes + (size_t)Np * realSize + perWave * slicesPerBlock;
 }
 
 template <class real>
 void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, hipStream_t stream) {
     SQ_THROW_IF(args.Np % 256 != 0, "padded N must be a multiple of 256");
-    SQ_THROW_IF(slicesPerBlock < 1 || slicesPerBlock > 16, "slices per block out of range");
-    SQ_THROW_IF(!plain && fieldFormat == fieldBF16x2 && sizeof(real) != 4, "bf16x2 fields need float");
+    SQ_THROW_IF(slicesPerBlock < 1 || slicesPerBlock > kMaxSlicesPerBlock, "slices per block out of range (1..%d)",
+                kMaxSlicesPerBlock);
     int nBlocks = divru(args.m, slicesPerBlock);
     size_t lds = denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain);
     int xcdRemap = (nBlocks % 8 == 0 && nBlocks >= 16) ? 1 : 0;
 
-    void *fn;
-    if (plain)
-        fn = (void *)denseSweepKernel<real, true, fieldF32>;
-    else if (fieldFormat == fieldBF16x2)
-        fn = (void *)denseSweepKernel<real, false, fieldBF16x2>;
-    else
-        fn = (void *)denseSweepKernel<real, false, fieldF32>;
+    (void)fieldFormat; /* the J rows passed in already carry the field format */
+    void *fn = plain ? (void *)denseSweepKernel<real, true> : (void *)denseSweepKernel<real, false>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SweepArgs<real> a = args;
     void *kargs[] = {(void *)&a, (void *)&slicesPerBlock, (void *)&xcdRemap};

@@ -64,8 +64,10 @@ enum FieldFormat {
     fieldBF16x2 = 1 /* J ~ hi + lo, both bf16, hi = bf16(J), lo = bf16(J - hi) */
 };
 
-/* Splits J (N x Np, float) into bf16 hi/lo planes of the same shape. */
-void splitBF16(const float *d_J, size_t count, uint16_t *d_hi, uint16_t *d_lo, hipStream_t stream);
+/* Splits J (N x Np, float) into bf16 hi/lo planes of the same shape, and
+ * writes Jsplit = float(hi) + float(lo) (exact in fp32): the J the sweep's
+ * row updates use, so that fields stay consistent with the split GEMM. */
+void splitBF16(const float *d_J, size_t count, uint16_t *d_hi, uint16_t *d_lo, float *d_Jsplit, hipStream_t stream);
 
 /* F[y][i] = sum_j q[y][j] * J[j][i] for all y < m, i < Np (J symmetric, N x Np
  * with zero padding), spins from packed bits.  real = float | double. */
@@ -84,8 +86,7 @@ struct SweepArgs {
     int sa;                 /* 0: SQA coloring, 1: SA (no Trotter coupling) */
     real twoDivM, coef, beta;
     real invKT;
-    const real *J;          /* field format fieldF32: N x Np */
-    const uint16_t *Jhi, *Jlo; /* field format fieldBF16x2: N x Np */
+    const real *J;          /* N x Np rows used by the sweep (J, or its bf16 hi+lo rounding) */
     const real *h;          /* Np */
     real *F;                /* m x Np, in/out */
     uint32_t *Qbits;        /* m x Nw, in/out */
@@ -101,7 +102,7 @@ struct SweepArgs {
 /* One annealOneStep sweep: N attempts per slice with exact serial-coloring
  * semantics.  plain = true computes each attempt's field as a J-row . q-row
  * dot (the reference algorithm); plain = false keeps the local fields in LDS.
- * fieldFormat applies to plain = false. */
+ * fieldFormat is informational: args.J already holds the rows of the form. */
 template <class real>
 void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, hipStream_t stream);
 

@@ -133,6 +133,11 @@ public:
 
     /* diagnostics (not in the reference API) */
     void getCounters(unsigned long long out[3]);
+    /* With the "profile" preference on, HIP events bracket each phase of
+     * annealOneStep on the solver's stream; this returns the summed device
+     * times (ms) and launch counts of [field refresh, stream generation,
+     * sweep] since the last call, and resets them. */
+    void getPhaseTimes(double ms[3], unsigned long long counts[3]);
     /* Max |E_real - E_fp64| of the last calculate_E (the rounding of the
      * FP64 recompute to real). */
     double lastEnergyRoundingError() const { return lastERoundErr_; }
@@ -162,12 +167,27 @@ private:
     int refreshInterval_ = 0;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
     int spinLimit_ = 1 << 20;
 
+    bool profile_ = false;        /* "profile": time phases with HIP events */
+    struct PhaseEvents {
+        int phase;
+        hipEvent_t start, stop;
+    };
+    std::vector<PhaseEvents> pending_;
+    std::vector<hipEvent_t> eventPool_;
+    double phaseMs_[3] = {0, 0, 0};
+    unsigned long long phaseCount_[3] = {0, 0, 0};
+    hipEvent_t takeEvent();
+    void phaseBegin(int phase, PhaseEvents &pe);
+    void phaseEnd(PhaseEvents &pe);
+    void collectPhases();
+
     bool fieldsDirty_ = true;
     int stepsSinceRefresh_ = 0;
     double lastERoundErr_ = 0.;
 
     /* device state */
     DevBuf<real> d_J_, d_h_, d_F_, d_U_;
+    DevBuf<float> d_Jsplit_;  /* bf16x2 form: float(hi) + float(lo) */
     DevBuf<uint16_t> d_Jhi_, d_Jlo_;
     DevBuf<uint32_t> d_Q_, d_X_, d_D_, d_words_;
     DevBuf<MTState> d_mt_;

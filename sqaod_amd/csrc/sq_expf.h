@@ -59,7 +59,8 @@ SQ_HD static inline double sq_u2d(uint64_t u) {
 #define SQ_EXPF_FUSED_POLY 1
 #endif
 
-SQ_HD static inline float sq_expf(float x) {
+/* tab: the 32-entry table above, or a copy of it (e.g. in GPU LDS). */
+SQ_HD static inline float sq_expf_tab(float x, const uint64_t *tab) {
 #ifdef __clang__
 #pragma clang fp contract(off)
 #endif
@@ -95,7 +96,7 @@ SQ_HD static inline float sq_expf(float x) {
     kd -= Shift;
     double r = z - kd;
 #endif
-    uint64_t t = sq_exp2_tab32[ki % 32];
+    uint64_t t = tab[ki % 32];
     t += ki << (52 - 5);
     double s = sq_u2d(t);
     double r2 = r * r;
@@ -111,3 +112,5 @@ SQ_HD static inline float sq_expf(float x) {
     y = y * s;
     return (float)y;
 }
+
+SQ_HD static inline float sq_expf(float x) { return sq_expf_tab(x, sq_exp2_tab32); }

@@ -54,6 +54,7 @@ _SIGS = {
     "sqaod_rocm_dg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
     "sqaod_rocm_dg_annealer_get_counters": (_i32, [_vp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dg_annealer_get_phase_times": (_i32, [_vp, ctypes.POINTER(_f64), ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -180,7 +181,8 @@ class _DenseGraphAnnealer:
             if not item:
                 continue
             k, v = item.split('=', 1)
-            out[k] = int(v) if k in ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval') else v
+            out[k] = int(v) if k in ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval',
+                                     'profile') else v
         return out
 
     @staticmethod
@@ -255,6 +257,13 @@ class _DenseGraphAnnealer:
         c = (ctypes.c_ulonglong * 3)()
         _check(_lib.sqaod_rocm_dg_annealer_get_counters(cobj, c))
         return {'flips': c[0], 'batches': c[1], 'blocked': c[2]}
+
+    @staticmethod
+    def get_phase_times(cobj, dtype):
+        ms = (_f64 * 3)()
+        cnt = (ctypes.c_ulonglong * 3)()
+        _check(_lib.sqaod_rocm_dg_annealer_get_phase_times(cobj, ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(("refresh", "rng", "sweep"))}
 
     @staticmethod
     def get_energy_rounding_error(cobj, dtype):

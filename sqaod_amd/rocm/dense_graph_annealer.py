@@ -21,6 +21,15 @@ class DenseGraphAnnealer(DenseGraphAnnealerBase):
         """Accepted flips, committed batches and blocked waits since prepare()."""
         return cext.get_counters(self._cobj, self.dtype)
 
+    def get_phase_times(self):
+        """With preference profile=1: {phase: (device ms, launches)} for the
+        refresh, rng and sweep phases since the last call."""
+        return cext.get_phase_times(self._cobj, self.dtype)
+
+    def get_energy_rounding_error(self):
+        """max |E(dtype) - E(fp64)| of the last energy evaluation."""
+        return cext.get_energy_rounding_error(self._cobj, self.dtype)
+
 
 def dense_graph_annealer(W=None, optimize=minimize, dtype=np.float64, **prefs):
     """Factory of sqaod_amd.rocm.DenseGraphAnnealer (same signature as
