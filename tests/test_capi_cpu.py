@@ -1,0 +1,108 @@
+"""The C-ABI library (include/sqaod_rocm.h) loads and exports every entry
+point the header declares, and the host-side solver logic (state machine,
+preferences, argument errors) behaves like the reference's without a GPU:
+every case here raises before any device call."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sqaod_rocm.h")
+
+
+@pytest.fixture(scope="module")
+def cext():
+    from sqaod_amd.rocm import _cext
+    return _cext
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sqaod_rocm_\w+)\s*\(", txt)))
+
+
+def test_header_symbols_exported(cext):
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(cext._lib, s), s
+    assert set(syms) == set(cext.EXPORTED)
+
+
+def test_version(cext):
+    assert cext.version() == (1, 0)
+
+
+def test_state_machine_errors_without_gpu(cext):
+    """Solver.cpp:96-112 messages; anneal before prepare raises
+    (test_dense_graph_annealer.py:390-393)."""
+    A = cext.rocm_dg_annealer
+    h = A.new(np.float32)
+    try:
+        with pytest.raises(RuntimeError, match="Bits\\(x or q\\) not initialized"):
+            A.anneal_one_step(h, 1.0, 1.0, np.float32)
+        with pytest.raises(RuntimeError, match="Problem is not set"):
+            A.prepare(h, np.float32)
+        with pytest.raises(RuntimeError, match="Problem is not set"):
+            A.get_problem_size(h, np.float32)
+        with pytest.raises(RuntimeError, match="positive integer"):
+            A.set_preferences(h, {"n_trotters": 0}, np.float32)
+        with pytest.raises(RuntimeError, match="unknown preference"):
+            A.set_preferences(h, {"no_such_pref": 1}, np.float32)
+        with pytest.raises(RuntimeError, match="unknown rng"):
+            A.set_preferences(h, {"rng": "xorshift"}, np.float32)
+        with pytest.raises(RuntimeError, match="not initialized"):
+            A.calculate_E(h, np.float32)
+    finally:
+        A.delete(h, np.float32)
+
+
+def test_preferences_defaults_and_algorithm_fallback(cext):
+    """Default algorithm coloring; naive maps to coloring and sa_default to
+    sa_naive, as the CUDA annealer does (CUDADenseGraphAnnealer.cu:115-126,
+    test_dense_graph_annealer.py:454-471)."""
+    A = cext.rocm_dg_annealer
+    h = A.new(np.float64)
+    try:
+        p = A.get_preferences(h, np.float64)
+        assert p["algorithm"] == "coloring" and p["device"] == "rocm" and p["precision"] == "double"
+        A.set_preferences(h, {"algorithm": "naive"}, np.float64)
+        assert A.get_preferences(h, np.float64)["algorithm"] == "coloring"
+        A.set_preferences(h, {"algorithm": "sa_default"}, np.float64)
+        assert A.get_preferences(h, np.float64)["algorithm"] == "sa_naive"
+        A.set_preferences(h, {"algorithm": "default", "n_trotters": 7}, np.float64)
+        p = A.get_preferences(h, np.float64)
+        assert p["algorithm"] == "coloring" and p["n_trotters"] == 7
+        with pytest.raises(RuntimeError, match="bf16x2"):
+            A.set_preferences(h, {"field_precision": "bf16x2"}, np.float64)
+    finally:
+        A.delete(h, np.float64)
+
+
+def test_bad_dtype_and_shapes(cext):
+    A = cext.rocm_dg_annealer
+    with pytest.raises(RuntimeError):
+        A.new(np.int32)
+    h = A.new(np.float32)
+    try:
+        with pytest.raises(RuntimeError, match="square"):
+            A.set_qubo(h, np.ones((3, 4), np.float32), 0, np.float32)
+        with pytest.raises(RuntimeError, match="optimize"):
+            A.set_qubo(h, np.ones((3, 3), np.float32), 5, np.float32)
+    finally:
+        A.delete(h, np.float32)
+
+
+def test_python_front_end_mirrors_reference_module():
+    """sqaod_amd mirrors the sqaod top level and the device package layout."""
+    import sqaod_amd as sq
+    import sqaod_amd.rocm as rocm
+    assert sq.algorithm.coloring == "coloring" and int(sq.minimize) == 0 and int(sq.maximize) == 1
+    assert callable(rocm.dense_graph_annealer)
+    x = sq.create_bitset_sequence([5], 4)
+    assert x.tolist() == [[0, 1, 0, 1]]
+    assert sq.bit_to_spin(x).tolist() == [[-1, 1, -1, 1]]
+    W = np.triu(np.ones((4, 4)))
+    assert np.array_equal(sq.symmetrize(W), (W + W.T) * 0.5)
