@@ -69,7 +69,9 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
  * "algorithm" ("coloring" | "sa_naive" | "naive" | "default" | ...),
  * "n_trotters" (int), and the rocm-device knobs "rng" ("philox" | "mt19937"),
  * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2"),
- * "slices_per_block" (int, 0 = auto), "refresh_interval" (int). */
+ * "slices_per_block" (int, 0 = auto), "refresh_interval" (int, steps between
+ * field recomputes; 0 = only after the spins were set), "refresh_engine"
+ * ("mfma" | "vector"), "profile" ("0" | "1"). */
 int sqaod_rocm_dg_annealer_set_preference(sqaod_rocm_dg_annealer *a, const char *name, const char *value);
 /* annealer_get_preferences (:302-329): "name=value" pairs joined by ';'.
  * Writes at most buflen bytes (NUL-terminated); *needed receives the full length + 1. */
@@ -105,6 +107,10 @@ int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, 
  * max |E_dtype - E_fp64| of the last energy evaluation. */
 int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]);
 int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err);
+/* Diagnostic without a reference counterpart: the local fields F[y][i] =
+ * sum_j J[i][j] q[y][j] (m x N, dtype) the local-field sweep holds, to check
+ * the field GEMM and its precision forms. */
+int sqaod_rocm_dg_annealer_get_fields(sqaod_rocm_dg_annealer *a, void *F, int m, int N);
 /* With preference "profile" = "1": device time (ms, HIP events on the solver's
  * stream) and launch counts of [field refresh, random-stream generation, sweep]
  * accumulated since the previous call; resets the accumulators. */

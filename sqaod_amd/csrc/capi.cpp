@@ -299,6 +299,20 @@ int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long
     return guarded([&] { withDg(a, [&](auto &s) { s.getCounters(counters); }); });
 }
 
+int sqaod_rocm_dg_annealer_get_fields(sqaod_rocm_dg_annealer *a, void *F, int m, int N) {
+    return guarded([&] {
+        SQ_THROW_IF(!a || !F, "null argument");
+        DgHandle *h = reinterpret_cast<DgHandle *>(a);
+        if (h->dtype == 0) {
+            SQ_THROW_IF(m != h->f->numTrotters() || N != h->f->getProblemSize(), "F buffer shape mismatch");
+            h->f->getFields((float *)F);
+        } else {
+            SQ_THROW_IF(m != h->d->numTrotters() || N != h->d->getProblemSize(), "F buffer shape mismatch");
+            h->d->getFields((double *)F);
+        }
+    });
+}
+
 int sqaod_rocm_dg_annealer_get_phase_times(sqaod_rocm_dg_annealer *a, double ms[3], unsigned long long counts[3]) {
     return guarded([&] { withDg(a, [&](auto &s) { s.getPhaseTimes(ms, counts); }); });
 }

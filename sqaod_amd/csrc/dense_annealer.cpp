@@ -311,6 +311,14 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
                           dev_->stream());
             }
         }
+    } else if (name == "refresh_engine") {
+        if (value == "mfma")
+            refreshVector_ = false;
+        else if (value == "vector")
+            refreshVector_ = true;
+        else
+            SQ_THROW("unknown refresh_engine '%s' (mfma | vector)", value.c_str());
+        fieldsDirty_ = true;
     } else if (name == "slices_per_block") {
         slicesPerBlock_ = atoi(value.c_str());
     } else if (name == "refresh_interval") {
@@ -341,6 +349,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("field_precision", fieldFormat_ == fieldBF16x2 ? "bf16x2" : (sizeof(real) == 4 ? "fp32" : "fp64"));
     p.emplace_back("slices_per_block", std::to_string(slicesPerBlock_));
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
+    p.emplace_back("refresh_engine", refreshVector_ ? "vector" : "mfma");
     p.emplace_back("profile", profile_ ? "1" : "0");
     return p;
 }
@@ -555,10 +564,19 @@ void DenseGraphAnnealer<real>::refreshFields() {
         fieldsDirty_ = false;
         return;
     }
-    if (fieldFormat_ == fieldBF16x2)
-        fieldRefreshBF16x2(d_Jhi_.ptr, d_Jlo_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr, dev_->stream());
-    else
+    /* F = Q . J in the field format: MFMA for float solvers, vector FMA for
+     * double (and on request, as the cross-check of the MFMA kernels) */
+    if (fieldFormat_ == fieldBF16x2) {
+        if (refreshVector_)
+            fieldRefreshBF16x2Vector(d_Jhi_.ptr, d_Jlo_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr,
+                                     dev_->stream());
+        else
+            fieldRefreshBF16x2(d_Jhi_.ptr, d_Jlo_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr, dev_->stream());
+    } else if (sizeof(real) == 4 && !refreshVector_) {
+        fieldRefreshF32MFMA((const float *)d_J_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr, dev_->stream());
+    } else {
         fieldRefresh<real>(d_J_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, d_F_.ptr, dev_->stream());
+    }
     fieldsDirty_ = false;
     stepsSinceRefresh_ = 0;
 }
@@ -730,6 +748,19 @@ real DenseGraphAnnealer<real>::getSystemE(real G, real beta) {
     if (om_ == optMaximize)
         E *= real(-1.);
     return E;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::getFields(real *F) {
+    throwErrorIfQNotSet();
+    SQ_THROW_IF(plainSweep_, "the plain sweep keeps no local fields");
+    ensureDevice();
+    if (fieldsDirty_)
+        refreshFields();
+    HIP_CHECK(hipMemcpy2DAsync(F, (size_t)N_ * sizeof(real), d_F_.ptr, (size_t)Np_ * sizeof(real),
+                               (size_t)N_ * sizeof(real), m_, hipMemcpyDeviceToHost, dev_->stream()));
+    dev_->synchronize();
+    checkAbort(true);
 }
 
 template <class real>

@@ -4,8 +4,9 @@
  * are seeded from (the CPU annealer recomputes the same J-row . q-row dot per
  * attempt, CPUDenseGraphAnnealer.cpp:257).
  *
- * This file holds the vector-ALU forms (fp32 / fp64 J, and bf16 hi+lo J
- * accumulated in fp32).  Each block owns 256 columns and a tile of slices;
+ * This file holds the vector-ALU forms (fp64 J for double solvers, and the
+ * fp32 / bf16 hi+lo forms kept as the reference the MFMA kernels of
+ * dense_field_mfma.hip are tested against).  Each block owns 256 columns and a tile of slices;
  * spins enter as sign flips of J, so the products are exact.
  */
 #include "common.h"
@@ -93,8 +94,8 @@ void fieldRefresh(const real *d_J, const uint32_t *d_Qbits, int N, int Np, int m
     HIP_CHECK(hipGetLastError());
 }
 
-void fieldRefreshBF16x2(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np, int m,
-                        int Nw, float *d_F, hipStream_t stream) {
+void fieldRefreshBF16x2Vector(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np,
+                              int m, int Nw, float *d_F, hipStream_t stream) {
     dim3 grid(Np / kCols, divru(m, kYTile));
     fieldRefreshKernel<float, fieldBF16x2><<<grid, kCols, 0, stream>>>(nullptr, d_hi, d_lo, d_Qbits, N, Np, m, Nw,
                                                                        d_F);

@@ -365,6 +365,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
                     site[t] = __shfl(s, (g + t) & (kWave - 1));
                 }
                 const int nDots = firstStop - g < kDotGroup ? firstStop - g : kDotGroup;
+#pragma unroll 4
                 for (int j = lane * 4; j < Np; j += kWave * 4) {
                     const uint32_t qw = Qb[j >> 5] >> (j & 31);
 #pragma unroll

@@ -74,7 +74,12 @@ void splitBF16(const float *d_J, size_t count, uint16_t *d_hi, uint16_t *d_lo, f
 template <class real>
 void fieldRefresh(const real *d_J, const uint32_t *d_Qbits, int N, int Np, int m, int Nw, real *d_F,
                   hipStream_t stream);
-/* Same with J given as bf16 hi + lo planes (fp32 accumulation, MFMA). */
+/* MFMA forms (dense_field_mfma.hip): J given as bf16 hi + lo planes, or fp32
+ * J on the fp32 MFMA; fp32 accumulation.  J has N rows, Np columns. */
+void fieldRefreshBF16x2Vector(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np,
+                              int m, int Nw, float *d_F, hipStream_t stream);
+void fieldRefreshF32MFMA(const float *d_J, const uint32_t *d_Qbits, int N, int Np, int m, int Nw, float *d_F,
+                         hipStream_t stream);
 void fieldRefreshBF16x2(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np, int m,
                         int Nw, float *d_F, hipStream_t stream);
 

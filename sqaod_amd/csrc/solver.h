@@ -138,6 +138,9 @@ public:
      * times (ms) and launch counts of [field refresh, stream generation,
      * sweep] since the last call, and resets them. */
     void getPhaseTimes(double ms[3], unsigned long long counts[3]);
+    /* The local fields F = J q_y (m x N, real) as the sweep holds them after
+     * the last step (recomputed first when the spins were set since). */
+    void getFields(real *F);
     /* Max |E_real - E_fp64| of the last calculate_E (the rounding of the
      * FP64 recompute to real). */
     double lastEnergyRoundingError() const { return lastERoundErr_; }
@@ -164,7 +167,8 @@ private:
     bool plainSweep_ = false;     /* "sweep": "local" (default) | "plain" */
     int fieldFormat_ = fieldF32;  /* "field_precision": "fp32" | "bf16x2" */
     int slicesPerBlock_ = 0;      /* "slices_per_block": 0 = automatic */
-    int refreshInterval_ = 0;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
+    int refreshInterval_ = 1;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
+    bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
     int spinLimit_ = 1 << 20;
 
     bool profile_ = false;        /* "profile": time phases with HIP events */

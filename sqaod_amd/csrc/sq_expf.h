@@ -12,7 +12,8 @@
  * rounded doubles, minus i<<47 so that the integer part of k can be added to
  * the exponent field directly) and 2^(r/32) from a cubic in r; everything is
  * evaluated in double precision and rounded to float once at the end.
- * The table is regenerated from first principles by oracle/gen_exp2_table.py.
+ * The table entries are recomputed from first principles (correctly rounded
+ * 2^(i/32) from 80-digit decimal arithmetic) by the CPU test suite.
  *
  * Fused multiply-adds are explicit (__builtin_fma), matching an FMA build of
  * that scheme; every other operation is a single IEEE double operation, so the

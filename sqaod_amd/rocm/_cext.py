@@ -54,6 +54,7 @@ _SIGS = {
     "sqaod_rocm_dg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
     "sqaod_rocm_dg_annealer_get_counters": (_i32, [_vp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dg_annealer_get_fields": (_i32, [_vp, _vp, _i32, _i32]),
     "sqaod_rocm_dg_annealer_get_phase_times": (_i32, [_vp, ctypes.POINTER(_f64), ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
 }
@@ -257,6 +258,13 @@ class _DenseGraphAnnealer:
         c = (ctypes.c_ulonglong * 3)()
         _check(_lib.sqaod_rocm_dg_annealer_get_counters(cobj, c))
         return {'flips': c[0], 'batches': c[1], 'blocked': c[2]}
+
+    @staticmethod
+    def get_fields(cobj, dtype):
+        m, N = _DenseGraphAnnealer._m(cobj), _DenseGraphAnnealer.get_problem_size(cobj, dtype)
+        F = np.empty((m, N), dtype)
+        _check(_lib.sqaod_rocm_dg_annealer_get_fields(cobj, _ptr(F), m, N))
+        return F
 
     @staticmethod
     def get_phase_times(cobj, dtype):

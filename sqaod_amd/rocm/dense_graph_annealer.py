@@ -26,6 +26,10 @@ class DenseGraphAnnealer(DenseGraphAnnealerBase):
         refresh, rng and sweep phases since the last call."""
         return cext.get_phase_times(self._cobj, self.dtype)
 
+    def get_fields(self):
+        """Local fields F = J q_y (m x N) of the local-field sweep."""
+        return cext.get_fields(self._cobj, self.dtype)
+
     def get_energy_rounding_error(self):
         """max |E(dtype) - E(fp64)| of the last energy evaluation."""
         return cext.get_energy_rounding_error(self._cobj, self.dtype)
