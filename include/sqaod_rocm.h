@@ -32,6 +32,8 @@ extern "C" {
 
 typedef struct sqaod_rocm_device sqaod_rocm_device;
 typedef struct sqaod_rocm_dg_annealer sqaod_rocm_dg_annealer;
+typedef struct sqaod_rocm_dg_bf_searcher sqaod_rocm_dg_bf_searcher;
+typedef struct sqaod_rocm_bg_annealer sqaod_rocm_bg_annealer;
 
 /* ---- library ---- */
 
@@ -115,6 +117,90 @@ int sqaod_rocm_dg_annealer_get_fields(sqaod_rocm_dg_annealer *a, void *F, int m,
  * stream) and launch counts of [field refresh, random-stream generation, sweep]
  * accumulated since the previous call; resets the accumulators. */
 int sqaod_rocm_dg_annealer_get_phase_times(sqaod_rocm_dg_annealer *a, double ms[3], unsigned long long counts[3]);
+
+/* ---- dense graph brute-force searcher: bf_searcher.inc method table (:479-494) ----
+ * Result: Emin = min over x in [0, 2^N) of x^T W x (maximize: max), and the
+ * first tile_size x reaching it in ascending order of the packed x (bit N-1-i
+ * of the packed x is x[i], Common.cpp:79-93) -- the sqaod.cpu one-worker
+ * result (CPUDenseGraphBatchSearch.cpp:26-50, CPUDenseGraphBFSearcher.cpp:
+ * 104-139), which the CUDA device's tile merge also gives. */
+/* bf_searcher_new / delete / assign_device (:479-482) */
+int sqaod_rocm_dg_bf_searcher_new(int dtype, sqaod_rocm_dg_bf_searcher **out);
+int sqaod_rocm_dg_bf_searcher_delete(sqaod_rocm_dg_bf_searcher *b);
+int sqaod_rocm_dg_bf_searcher_assign_device(sqaod_rocm_dg_bf_searcher *b, sqaod_rocm_device *dev);
+/* bf_searcher_set_qubo (:75-104): W N x N symmetric, N <= 63; optimize 0 = minimize, 1 = maximize */
+int sqaod_rocm_dg_bf_searcher_set_qubo(sqaod_rocm_dg_bf_searcher *b, const void *W, int N, int optimize);
+/* bf_searcher_get_problem_size (:107-128) */
+int sqaod_rocm_dg_bf_searcher_get_problem_size(sqaod_rocm_dg_bf_searcher *b, int *N);
+/* bf_searcher_set_preferences (:486): "tile_size" (rounded up to a multiple of
+ * 256, Solver.cpp:201-209; caps the solution list), "algorithm" (always
+ * brute_force_search) */
+int sqaod_rocm_dg_bf_searcher_set_preference(sqaod_rocm_dg_bf_searcher *b, const char *name, const char *value);
+/* bf_searcher_get_preferences (:487), format as for the annealer */
+int sqaod_rocm_dg_bf_searcher_get_preferences(sqaod_rocm_dg_bf_searcher *b, char *buf, size_t buflen,
+                                              size_t *needed);
+/* bf_searcher_prepare / calculate_E / make_solution (:490-492) */
+int sqaod_rocm_dg_bf_searcher_prepare(sqaod_rocm_dg_bf_searcher *b);
+int sqaod_rocm_dg_bf_searcher_calculate_E(sqaod_rocm_dg_bf_searcher *b);
+int sqaod_rocm_dg_bf_searcher_make_solution(sqaod_rocm_dg_bf_searcher *b);
+/* bf_searcher_search_range (:424-445): scans the next span of x on the device
+ * (max(tile_size, 2^26) candidates; the reference's spans are one tile, or one
+ * tile per CPU worker); *done = 1 once all 2^N are scanned, *cur_x_end = end */
+int sqaod_rocm_dg_bf_searcher_search_range(sqaod_rocm_dg_bf_searcher *b, int *done, unsigned long long *cur_x_end);
+/* bf_searcher_search (:456-473): prepare, scan everything, make_solution */
+int sqaod_rocm_dg_bf_searcher_search(sqaod_rocm_dg_bf_searcher *b);
+/* number of solutions (rows of get_x); get_E has max(1, n) entries */
+int sqaod_rocm_dg_bf_searcher_get_n_solutions(sqaod_rocm_dg_bf_searcher *b, int *n);
+/* bf_searcher_get_x (:241-275): x[n x N] int8 */
+int sqaod_rocm_dg_bf_searcher_get_x(sqaod_rocm_dg_bf_searcher *b, int8_t *x, int n, int N);
+/* bf_searcher_get_E (:318-341): E[n] of dtype, n = max(1, number of solutions) */
+int sqaod_rocm_dg_bf_searcher_get_E(sqaod_rocm_dg_bf_searcher *b, void *E, int n);
+
+/* ---- bipartite graph annealer: annealer.inc method table (:885-908), the
+ * BIPARTITE_GRAPH variants.  W is N1 x N0 (rows index the x1 side), J likewise;
+ * q/x pairs are passed as two arrays (m x N0 and m x N1 int8). ---- */
+/* annealer_new / delete / assign_device / seed (:14-105) */
+int sqaod_rocm_bg_annealer_new(int dtype, sqaod_rocm_bg_annealer **out);
+int sqaod_rocm_bg_annealer_delete(sqaod_rocm_bg_annealer *a);
+int sqaod_rocm_bg_annealer_assign_device(sqaod_rocm_bg_annealer *a, sqaod_rocm_device *dev);
+int sqaod_rocm_bg_annealer_seed(sqaod_rocm_bg_annealer *a, unsigned long long seed);
+/* annealer_set_qubo (:108-141, bipartite form): b0[N0], b1[N1], W[N1 x N0] */
+int sqaod_rocm_bg_annealer_set_qubo(sqaod_rocm_bg_annealer *a, const void *b0, int N0, const void *b1, int N1,
+                                    const void *W, int optimize);
+/* annealer_set_hamiltonian (:144-160): h0[N0], h1[N1], J[N1 x N0], c scalar */
+int sqaod_rocm_bg_annealer_set_hamiltonian(sqaod_rocm_bg_annealer *a, const void *h0, int N0, const void *h1, int N1,
+                                           const void *J, const void *c);
+/* annealer_get_problem_size (:163-196): (N0, N1) */
+int sqaod_rocm_bg_annealer_get_problem_size(sqaod_rocm_bg_annealer *a, int *N0, int *N1);
+/* annealer_set_preferences / get_preferences (:279-329): "algorithm"
+ * ("coloring" | "sa_coloring", other names fall back as on the CUDA device),
+ * "n_trotters", "rng" ("philox" | "mt19937"), "field_precision" ("fp32" |
+ * "bf16x2"), "refresh_engine" ("mfma" | "vector") */
+int sqaod_rocm_bg_annealer_set_preference(sqaod_rocm_bg_annealer *a, const char *name, const char *value);
+int sqaod_rocm_bg_annealer_get_preferences(sqaod_rocm_bg_annealer *a, char *buf, size_t buflen, size_t *needed);
+/* number of Trotter slices m */
+int sqaod_rocm_bg_annealer_get_n_trotters(sqaod_rocm_bg_annealer *a, int *m);
+/* annealer_get_E (:332-364): E[m] of dtype */
+int sqaod_rocm_bg_annealer_get_E(sqaod_rocm_bg_annealer *a, void *E, int m);
+/* annealer_get_x (:367-392): x0[m x N0], x1[m x N1] int8 */
+int sqaod_rocm_bg_annealer_get_x(sqaod_rocm_bg_annealer *a, int8_t *x0, int8_t *x1, int m, int N0, int N1);
+/* annealer_get_hamiltonian (:470-503): h0[N0], h1[N1], J[N1 x N0], c[1] of dtype */
+int sqaod_rocm_bg_annealer_get_hamiltonian(sqaod_rocm_bg_annealer *a, void *h0, void *h1, void *J, void *c);
+/* annealer_get_q (:506-548): q0[m x N0], q1[m x N1] int8 */
+int sqaod_rocm_bg_annealer_get_q(sqaod_rocm_bg_annealer *a, int8_t *q0, int8_t *q1, int m, int N0, int N1);
+/* annealer_set_q (:395-431, (q0, q1) tuple): one pair for every slice */
+int sqaod_rocm_bg_annealer_set_q(sqaod_rocm_bg_annealer *a, const int8_t *q0, int N0, const int8_t *q1, int N1);
+/* annealer_set_qset (:434-467): m pairs, sets m */
+int sqaod_rocm_bg_annealer_set_qset(sqaod_rocm_bg_annealer *a, const int8_t *q0, const int8_t *q1, int m, int N0,
+                                    int N1);
+/* annealer_radomize_q / calculate_E / prepare / make_solution (:747-833) */
+int sqaod_rocm_bg_annealer_randomize_spin(sqaod_rocm_bg_annealer *a);
+int sqaod_rocm_bg_annealer_calculate_E(sqaod_rocm_bg_annealer *a);
+int sqaod_rocm_bg_annealer_prepare(sqaod_rocm_bg_annealer *a);
+int sqaod_rocm_bg_annealer_make_solution(sqaod_rocm_bg_annealer *a);
+/* annealer_get_system_E (:836-850), annealer_anneal_one_step (:853-878) */
+int sqaod_rocm_bg_annealer_get_system_E(sqaod_rocm_bg_annealer *a, double G, double beta, double *E);
+int sqaod_rocm_bg_annealer_anneal_one_step(sqaod_rocm_bg_annealer *a, double G, double beta);
 
 /* Self check without a reference counterpart: order-independent checksum of
  * the device exp (the acceptance test's expf) over float bit patterns

@@ -6,6 +6,7 @@
  */
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "sqaod_rocm.h"
 #include "solver.h"
@@ -44,6 +45,66 @@ void withDg(sqaod_rocm_dg_annealer *h, Fn &&fn) {
         fn(*dh->f);
     else
         fn(*dh->d);
+}
+
+struct BfHandle {
+    int dtype;
+    DenseGraphBFSearcher<float> *f = nullptr;
+    DenseGraphBFSearcher<double> *d = nullptr;
+};
+
+template <class Fn>
+void withBf(sqaod_rocm_dg_bf_searcher *h, Fn &&fn) {
+    SQ_THROW_IF(h == nullptr, "null searcher handle");
+    BfHandle *bh = reinterpret_cast<BfHandle *>(h);
+    if (bh->dtype == 0)
+        fn(*bh->f);
+    else
+        fn(*bh->d);
+}
+
+struct BgHandle {
+    int dtype;
+    BipartiteGraphAnnealer<float> *f = nullptr;
+    BipartiteGraphAnnealer<double> *d = nullptr;
+};
+
+template <class Fn>
+void withBg(sqaod_rocm_bg_annealer *h, Fn &&fn) {
+    SQ_THROW_IF(h == nullptr, "null annealer handle");
+    BgHandle *bh = reinterpret_cast<BgHandle *>(h);
+    if (bh->dtype == 0)
+        fn(*bh->f);
+    else
+        fn(*bh->d);
+}
+
+template <class S>
+void checkBgShape(S &s, int m, int N0, int N1, const char *what) {
+    int n0 = 0, n1 = 0;
+    s.getProblemSize(&n0, &n1);
+    SQ_THROW_IF(m != s.numTrotters() || N0 != n0 || N1 != n1, "%s: expected m=%d, N0=%d, N1=%d, got %d, %d, %d", what,
+                s.numTrotters(), n0, n1, m, N0, N1);
+}
+
+std::string joinPrefs(const PrefList &prefs) {
+    std::string out;
+    for (auto &kv : prefs) {
+        if (!out.empty())
+            out += ';';
+        out += kv.first + "=" + kv.second;
+    }
+    return out;
+}
+
+void copyOut(const std::string &out, char *buf, size_t buflen, size_t *needed) {
+    if (needed)
+        *needed = out.size() + 1;
+    if (buf && buflen) {
+        size_t n = out.size() < buflen - 1 ? out.size() : buflen - 1;
+        memcpy(buf, out.data(), n);
+        buf[n] = '\0';
+    }
 }
 
 }  // namespace
@@ -326,6 +387,307 @@ int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsig
         SQ_THROW_IF(!out || lo > hi || hi > 0x100000000ull, "bad range");
         defaultDevice().makeCurrent();
         *out = expfChecksum(lo, hi);
+    });
+}
+
+/* ---------------- dense graph brute-force searcher ---------------- */
+
+int sqaod_rocm_dg_bf_searcher_new(int dtype, sqaod_rocm_dg_bf_searcher **out) {
+    return guarded([&] {
+        SQ_THROW_IF(out == nullptr, "null output pointer");
+        SQ_THROW_IF(dtype != 0 && dtype != 1, "dtype must be 0 (float32) or 1 (float64)");
+        auto *h = new BfHandle;
+        h->dtype = dtype;
+        if (dtype == 0)
+            h->f = new DenseGraphBFSearcher<float>();
+        else
+            h->d = new DenseGraphBFSearcher<double>();
+        *out = reinterpret_cast<sqaod_rocm_dg_bf_searcher *>(h);
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_delete(sqaod_rocm_dg_bf_searcher *b) {
+    return guarded([&] {
+        if (!b)
+            return;
+        BfHandle *h = reinterpret_cast<BfHandle *>(b);
+        delete h->f;
+        delete h->d;
+        delete h;
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_assign_device(sqaod_rocm_dg_bf_searcher *b, sqaod_rocm_device *dev) {
+    return guarded([&] {
+        SQ_THROW_IF(!dev, "null device");
+        withBf(b, [&](auto &s) { s.assignDevice(*dev->dev); });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_set_qubo(sqaod_rocm_dg_bf_searcher *b, const void *W, int N, int optimize) {
+    return guarded([&] {
+        SQ_THROW_IF(!W, "null W");
+        withBf(b, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            s.setQUBO(static_cast<const real *>(W), N, optimize ? optMaximize : optMinimize);
+        });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_get_problem_size(sqaod_rocm_dg_bf_searcher *b, int *N) {
+    return guarded([&] {
+        withBf(b, [&](auto &s) {
+            s.throwErrorIfProblemNotSet();
+            *N = s.getProblemSize();
+        });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_set_preference(sqaod_rocm_dg_bf_searcher *b, const char *name, const char *value) {
+    return guarded([&] {
+        SQ_THROW_IF(!name || !value, "null preference");
+        withBf(b, [&](auto &s) { s.setPreference(name, value); });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_get_preferences(sqaod_rocm_dg_bf_searcher *b, char *buf, size_t buflen,
+                                              size_t *needed) {
+    return guarded([&] {
+        std::string out;
+        withBf(b, [&](auto &s) { out = joinPrefs(s.getPreferences()); });
+        copyOut(out, buf, buflen, needed);
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_prepare(sqaod_rocm_dg_bf_searcher *b) {
+    return guarded([&] { withBf(b, [&](auto &s) { s.prepare(); }); });
+}
+
+int sqaod_rocm_dg_bf_searcher_calculate_E(sqaod_rocm_dg_bf_searcher *b) {
+    return guarded([&] { withBf(b, [&](auto &s) { s.calculate_E(); }); });
+}
+
+int sqaod_rocm_dg_bf_searcher_make_solution(sqaod_rocm_dg_bf_searcher *b) {
+    return guarded([&] { withBf(b, [&](auto &s) { s.makeSolution(); }); });
+}
+
+int sqaod_rocm_dg_bf_searcher_search_range(sqaod_rocm_dg_bf_searcher *b, int *done, unsigned long long *cur_x_end) {
+    return guarded([&] {
+        withBf(b, [&](auto &s) {
+            unsigned long long x = 0;
+            bool d = s.searchRange(&x);
+            if (done)
+                *done = d ? 1 : 0;
+            if (cur_x_end)
+                *cur_x_end = x;
+        });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_search(sqaod_rocm_dg_bf_searcher *b) {
+    return guarded([&] { withBf(b, [&](auto &s) { s.search(); }); });
+}
+
+int sqaod_rocm_dg_bf_searcher_get_n_solutions(sqaod_rocm_dg_bf_searcher *b, int *n) {
+    return guarded([&] {
+        SQ_THROW_IF(!n, "null output pointer");
+        withBf(b, [&](auto &s) { *n = s.numSolutions(); });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_get_x(sqaod_rocm_dg_bf_searcher *b, int8_t *x, int n, int N) {
+    return guarded([&] {
+        withBf(b, [&](auto &s) {
+            const int ns = s.numSolutions();
+            SQ_THROW_IF(n != ns || N != s.getProblemSize(), "get_x: expected %d x %d, got %d x %d", ns,
+                        s.getProblemSize(), n, N);
+            if (ns)
+                s.get_x(x);
+        });
+    });
+}
+
+int sqaod_rocm_dg_bf_searcher_get_E(sqaod_rocm_dg_bf_searcher *b, void *E, int n) {
+    return guarded([&] {
+        withBf(b, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            const int ne = s.numEnergies();
+            SQ_THROW_IF(n != ne, "get_E: expected %d entries, got %d", ne, n);
+            s.get_E(static_cast<real *>(E));
+        });
+    });
+}
+
+/* ---------------- bipartite graph annealer ---------------- */
+
+int sqaod_rocm_bg_annealer_new(int dtype, sqaod_rocm_bg_annealer **out) {
+    return guarded([&] {
+        SQ_THROW_IF(out == nullptr, "null output pointer");
+        SQ_THROW_IF(dtype != 0 && dtype != 1, "dtype must be 0 (float32) or 1 (float64)");
+        auto *h = new BgHandle;
+        h->dtype = dtype;
+        if (dtype == 0)
+            h->f = new BipartiteGraphAnnealer<float>();
+        else
+            h->d = new BipartiteGraphAnnealer<double>();
+        *out = reinterpret_cast<sqaod_rocm_bg_annealer *>(h);
+    });
+}
+
+int sqaod_rocm_bg_annealer_delete(sqaod_rocm_bg_annealer *a) {
+    return guarded([&] {
+        if (!a)
+            return;
+        BgHandle *h = reinterpret_cast<BgHandle *>(a);
+        delete h->f;
+        delete h->d;
+        delete h;
+    });
+}
+
+int sqaod_rocm_bg_annealer_assign_device(sqaod_rocm_bg_annealer *a, sqaod_rocm_device *dev) {
+    return guarded([&] {
+        SQ_THROW_IF(!dev, "null device");
+        withBg(a, [&](auto &s) { s.assignDevice(*dev->dev); });
+    });
+}
+
+int sqaod_rocm_bg_annealer_seed(sqaod_rocm_bg_annealer *a, unsigned long long seed) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.seed(seed); }); });
+}
+
+int sqaod_rocm_bg_annealer_set_qubo(sqaod_rocm_bg_annealer *a, const void *b0, int N0, const void *b1, int N1,
+                                    const void *W, int optimize) {
+    return guarded([&] {
+        SQ_THROW_IF(!b0 || !b1 || !W, "null input");
+        withBg(a, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            s.setQUBO(static_cast<const real *>(b0), static_cast<const real *>(b1), static_cast<const real *>(W), N0,
+                      N1, optimize ? optMaximize : optMinimize);
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_set_hamiltonian(sqaod_rocm_bg_annealer *a, const void *h0, int N0, const void *h1, int N1,
+                                           const void *J, const void *c) {
+    return guarded([&] {
+        SQ_THROW_IF(!h0 || !h1 || !J || !c, "null input");
+        withBg(a, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            s.setHamiltonian(static_cast<const real *>(h0), static_cast<const real *>(h1),
+                             static_cast<const real *>(J), *static_cast<const real *>(c), N0, N1);
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_problem_size(sqaod_rocm_bg_annealer *a, int *N0, int *N1) {
+    return guarded([&] {
+        SQ_THROW_IF(!N0 || !N1, "null output pointer");
+        withBg(a, [&](auto &s) { s.getProblemSize(N0, N1); });
+    });
+}
+
+int sqaod_rocm_bg_annealer_set_preference(sqaod_rocm_bg_annealer *a, const char *name, const char *value) {
+    return guarded([&] {
+        SQ_THROW_IF(!name || !value, "null preference");
+        withBg(a, [&](auto &s) { s.setPreference(name, value); });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_preferences(sqaod_rocm_bg_annealer *a, char *buf, size_t buflen, size_t *needed) {
+    return guarded([&] {
+        std::string out;
+        withBg(a, [&](auto &s) { out = joinPrefs(s.getPreferences()); });
+        copyOut(out, buf, buflen, needed);
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_n_trotters(sqaod_rocm_bg_annealer *a, int *m) {
+    return guarded([&] {
+        SQ_THROW_IF(!m, "null output pointer");
+        withBg(a, [&](auto &s) { *m = s.numTrotters(); });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_E(sqaod_rocm_bg_annealer *a, void *E, int m) {
+    return guarded([&] {
+        withBg(a, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            SQ_THROW_IF(m != s.numTrotters(), "get_E: expected %d entries, got %d", s.numTrotters(), m);
+            s.get_E(static_cast<real *>(E));
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_x(sqaod_rocm_bg_annealer *a, int8_t *x0, int8_t *x1, int m, int N0, int N1) {
+    return guarded([&] {
+        withBg(a, [&](auto &s) {
+            checkBgShape(s, m, N0, N1, "get_x");
+            s.get_x(x0, x1);
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_hamiltonian(sqaod_rocm_bg_annealer *a, void *h0, void *h1, void *J, void *c) {
+    return guarded([&] {
+        withBg(a, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            s.getHamiltonian(static_cast<real *>(h0), static_cast<real *>(h1), static_cast<real *>(J),
+                             static_cast<real *>(c));
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_q(sqaod_rocm_bg_annealer *a, int8_t *q0, int8_t *q1, int m, int N0, int N1) {
+    return guarded([&] {
+        withBg(a, [&](auto &s) {
+            checkBgShape(s, m, N0, N1, "get_q");
+            s.get_q(q0, q1);
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_set_q(sqaod_rocm_bg_annealer *a, const int8_t *q0, int N0, const int8_t *q1, int N1) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.set_q(q0, N0, q1, N1); }); });
+}
+
+int sqaod_rocm_bg_annealer_set_qset(sqaod_rocm_bg_annealer *a, const int8_t *q0, const int8_t *q1, int m, int N0,
+                                    int N1) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.set_qset(q0, q1, m, N0, N1); }); });
+}
+
+int sqaod_rocm_bg_annealer_randomize_spin(sqaod_rocm_bg_annealer *a) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.randomizeSpin(); }); });
+}
+
+int sqaod_rocm_bg_annealer_calculate_E(sqaod_rocm_bg_annealer *a) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.calculate_E(); }); });
+}
+
+int sqaod_rocm_bg_annealer_prepare(sqaod_rocm_bg_annealer *a) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.prepare(); }); });
+}
+
+int sqaod_rocm_bg_annealer_make_solution(sqaod_rocm_bg_annealer *a) {
+    return guarded([&] { withBg(a, [&](auto &s) { s.makeSolution(); }); });
+}
+
+int sqaod_rocm_bg_annealer_get_system_E(sqaod_rocm_bg_annealer *a, double G, double beta, double *E) {
+    return guarded([&] {
+        SQ_THROW_IF(!E, "null output pointer");
+        withBg(a, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            *E = (double)s.getSystemE((real)G, (real)beta);
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_anneal_one_step(sqaod_rocm_bg_annealer *a, double G, double beta) {
+    return guarded([&] {
+        withBg(a, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            s.annealOneStep((real)G, (real)beta);
+        });
     });
 }
 

@@ -302,13 +302,11 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
         if (f != fieldFormat_) {
             fieldFormat_ = f;
             fieldsDirty_ = true;
+            fragValid_ = false;
             if (f == fieldBF16x2 && d_J_.ptr) {
                 ensureDevice();
-                d_Jhi_.allocate((size_t)N_ * Np_);
-                d_Jlo_.allocate((size_t)N_ * Np_);
                 d_Jsplit_.allocate((size_t)N_ * Np_);
-                splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, d_Jsplit_.ptr,
-                          dev_->stream());
+                splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jsplit_.ptr, dev_->stream());
             }
         }
     } else if (name == "refresh_engine") {
@@ -373,11 +371,10 @@ void DenseGraphAnnealer<real>::uploadJ(const real *J, const real *h, real c) {
                                (size_t)N_ * sizeof(real), N_, hipMemcpyHostToDevice, dev_->stream()));
     HIP_CHECK(hipMemcpyAsync(d_h_.ptr, h, (size_t)N_ * sizeof(real), hipMemcpyHostToDevice, dev_->stream()));
     if (fieldFormat_ == fieldBF16x2) {
-        d_Jhi_.allocate((size_t)N_ * Np_);
-        d_Jlo_.allocate((size_t)N_ * Np_);
         d_Jsplit_.allocate((size_t)N_ * Np_);
-        splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jhi_.ptr, d_Jlo_.ptr, d_Jsplit_.ptr, dev_->stream());
+        splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jsplit_.ptr, dev_->stream());
     }
+    fragValid_ = false;
     c_ = c;
     h_.assign(h, h + N_);
     dev_->synchronize();
@@ -502,7 +499,7 @@ void DenseGraphAnnealer<real>::randomizeSpin() {
     ensureDevice();
     if (rngMT_) {
         mtGenerate(d_mt_.ptr, d_words_.ptr, (size_t)m_ * N_, dev_->stream());
-        spinsFromWords(d_words_.ptr, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
+        spinsFromWords(d_words_.ptr, N_, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
     } else {
         spinsFromPhilox(seed_, sweepIdx_++, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
     }
@@ -566,16 +563,34 @@ void DenseGraphAnnealer<real>::refreshFields() {
     }
     /* F = Q . J in the field format: MFMA for float solvers, vector FMA for
      * double (and on request, as the cross-check of the MFMA kernels) */
+    const bool mfma = sizeof(real) == 4 && !refreshVector_;
+    if (mfma && !fragValid_) {
+        /* J in MFMA fragment order, built once per problem and form */
+        const float *J = (const float *)d_J_.ptr;
+        if (fieldFormat_ == fieldBF16x2) {
+            d_frag32_.release();
+            d_frag16_.allocate(fieldFragElemsBF16x2(Np_, Np_));
+            buildFragBF16x2(J, N_, Np_, Np_, Np_, d_frag16_.ptr, d_frag16_.n, dev_->stream());
+        } else {
+            d_frag16_.release();
+            d_frag32_.allocate(fieldFragElemsF32(Np_, Np_));
+            buildFragF32(J, N_, Np_, Np_, Np_, d_frag32_.ptr, d_frag32_.n, dev_->stream());
+        }
+        fragValid_ = true;
+    }
     if (fieldFormat_ == fieldBF16x2) {
-        if (refreshVector_)
-            fieldRefreshBF16x2Vector(d_Jhi_.ptr, d_Jlo_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr,
-                                     dev_->stream());
+        /* the vector cross-check runs on Jsplit = float(hi) + float(lo), the
+         * same numbers as the two planes */
+        if (mfma)
+            fieldRefreshBF16x2(d_frag16_.ptr, d_frag16_.n, d_Q_.ptr, Np_, Np_, m_, Nw_, Np_, (float *)d_F_.ptr,
+                               dev_->stream());
         else
-            fieldRefreshBF16x2(d_Jhi_.ptr, d_Jlo_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr, dev_->stream());
-    } else if (sizeof(real) == 4 && !refreshVector_) {
-        fieldRefreshF32MFMA((const float *)d_J_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, (float *)d_F_.ptr, dev_->stream());
+            fieldRefresh<float>(d_Jsplit_.ptr, N_, Np_, Np_, d_Q_.ptr, m_, Nw_, Np_, (float *)d_F_.ptr, dev_->stream());
+    } else if (mfma) {
+        fieldRefreshF32MFMA(d_frag32_.ptr, d_frag32_.n, d_Q_.ptr, Np_, Np_, m_, Nw_, Np_, (float *)d_F_.ptr,
+                            dev_->stream());
     } else {
-        fieldRefresh<real>(d_J_.ptr, d_Q_.ptr, N_, Np_, m_, Nw_, d_F_.ptr, dev_->stream());
+        fieldRefresh<real>(d_J_.ptr, N_, Np_, Np_, d_Q_.ptr, m_, Nw_, Np_, d_F_.ptr, dev_->stream());
     }
     fieldsDirty_ = false;
     stepsSinceRefresh_ = 0;

@@ -1,23 +1,41 @@
 /*
- * Local-field refresh F = Q . J on the MFMA units (gfx950):
- *   F[y][i] = sum_k q[y][k] J[k][i] = sum_k q[y][k] J[i][k]   (J symmetric)
- * the GEMM (m x N) . (N x Np) that seeds the sweep's LDS fields.
+ * Spin-matrix GEMM on the MFMA units (gfx950):
+ *   F[y][c] = sum_k q[y][k] B[c][k]
+ * with q the +-1 spins of m slices (packed bits, K per row) and B given by
+ * rows.  The dense annealer's local-field refresh is B = J (symmetric, so
+ * F = Q . J); the bipartite annealer's half-steps are B = J and B = J^T
+ * (dEmat = qFixed . J^T, CPUBipartiteGraphAnnealer.cpp:351-352).
  *
  *  - bf16x2: J = hi + lo (two bf16 planes), spins +-1 exact in bf16,
  *    v_mfma_f32_16x16x32_bf16 twice per k-block, fp32 accumulation.
  *  - fp32:   v_mfma_f32_16x16x4_f32, spins as +-1.0f.
  *
- * Tiling: a block of 4 waves owns 16 output columns i; wave w owns the
- * slice rows [64 w, 64 w + 64) (+ 256 per further pass), i.e. four 16 x 16
- * accumulator tiles.  B fragments come straight from J's rows (symmetry makes
- * the k-run of column i contiguous in row i); A fragments are built from the
- * packed spin bits in registers.  The k dimension is assigned to lane groups
- * identically for A and B, so any consistent permutation of k is valid.
- * The k-blocks past N read zero padding (rows of J are padded to Np, and the
- * spin bits past N are zero), so ragged N needs no special case.
+ * J is read from a copy laid out in MFMA fragment order, built once per
+ * problem: for column block cb (16 columns) and k-block kb, the 64 lanes'
+ * B fragments are one contiguous run (1 KiB per plane), so every operand
+ * load of a wave is a single fully coalesced 1 KiB request and a block
+ * streams its columns' k-range front to back.
+ *
+ * Work split: one block of 4 waves owns 16 output columns and all slice rows
+ * of a pass (up to 256: sixteen 16 x 16 accumulator tiles per wave).  The
+ * k-range is split over the 4 waves in super-blocks of kDepth k-blocks
+ * (interleaved), each wave double-buffers its operand loads, and the four
+ * partial sums are added in a fixed order through LDS, so results are
+ * deterministic.  Each block starts its k-range at a different super-block,
+ * so concurrently running blocks read different address ranges.
+ *
+ * Spins: lane L loads the packed spin words of slice row y0 + 64 r + L for a
+ * whole super-block (one 16-byte load per row group); the A fragment of tile
+ * t is then fetched from the owning lane with ds_bpermute (__shfl).
+ *
+ * The k-blocks past the valid length read zero padding (rows of B are padded
+ * to K with zeros, the fragment copy is zero for columns past B's rows, and
+ * spin bits past the valid length are zero), so ragged shapes need no special
+ * case.
  */
 #include "common.h"
 #include "kernels.h"
+#include "bf16.h"
 
 namespace sqaod_rocm {
 
@@ -25,64 +43,81 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef short i16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kWave = 64;
-constexpr int kRowsPerWave = 64; /* four 16-row tiles */
-constexpr int kWavesPerBlock = 4;
+constexpr int kTiles = 16;        /* 16-row accumulator tiles per wave: 256 slice rows per pass */
+constexpr int kRowsPerPass = kTiles * 16;
+constexpr int kSplitK = 4;        /* waves per block, each a share of the k-range */
+constexpr int kDepth = 4;         /* k-blocks per super-block (the double-buffered unit) */
+constexpr int kBlockThreads = kSplitK * kWave;
 
-/* 8 spins (bits b0..b7 of `bits`) as bf16 +-1 */
+/* 8 spins (bits b0..b7 of `bits`) as bf16 +-1: each dword packs two
+ * elements, 0xBF80 (-1) with the sign bit cleared where the spin bit is 1. */
 __device__ __forceinline__ bf16x8 spinsBF16(uint32_t bits) {
-    i16x8 v;
+    uint32_t v[4];
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-        v[t] = ((bits >> t) & 1u) ? (short)0x3F80 : (short)0xBF80;
+    for (int j = 0; j < 4; ++j)
+        v[j] = 0xBF80BF80u ^ (((bits >> (2 * j)) & 1u) << 15) ^ ((bits >> (2 * j + 1)) << 31);
     return __builtin_bit_cast(bf16x8, v);
 }
 
-/* Operand fetch is double-buffered in super-blocks of kDepth k-blocks: the
- * loads of super-block s+1 are issued before the MFMAs of super-block s, so
- * the global-memory latency overlaps the matrix work. */
-constexpr int kDepth = 4;
-
-__device__ __forceinline__ uint32_t spinBits(const uint32_t *Qbits, int y, int m, int Nw, int kk, uint32_t mask) {
-    return (y < m) ? ((Qbits[(long long)y * Nw + (kk >> 5)] >> (kk & 31)) & mask) : 0u;
+/* Fixed-order sum of the kSplitK partial tiles in LDS and the store of the
+ * pass's rows: red[w][4 t + r][lane] holds wave w's acc[t][r]. */
+__device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int y0, int m, int cb, int ldF, float *F) {
+    __syncthreads();
+#pragma unroll 4
+    for (int j = 0; j < kTiles * 4 * kWave / kBlockThreads; ++j) {
+        const int o = j * kBlockThreads + threadIdx.x;
+        const int tr = o / kWave, L = o % kWave;
+        const int y = y0 + 16 * (tr >> 2) + 4 * (L >> 4) + (tr & 3);
+        if (y < m) {
+            float s = red[0][tr][L];
+#pragma unroll
+            for (int w = 1; w < kSplitK; ++w)
+                s += red[w][tr][L];
+            F[(long long)y * ldF + cb * 16 + (L & 15)] = s;
+        }
+    }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(kWavesPerBlock *kWave) void fieldRefreshBF16x2Kernel(const uint16_t *Jhi,
-                                                                                  const uint16_t *Jlo,
-                                                                                  const uint32_t *Qbits, int N,
-                                                                                  int Np, int m, int Nw, float *F) {
-    constexpr int KB = 32; /* k per block: one 16x16x32 MFMA per plane */
-    const int lane = threadIdx.x & (kWave - 1);
+/* frag: [column blocks][K/32 k-blocks][2 planes][64 lanes] x 16 B; block cb
+ * writes columns [16 cb, 16 cb + 16) of F (row stride ldF) */
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const uint4 *frag, const uint32_t *Qbits,
+                                                                          int K, int m, int Nw, int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
+    const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
-    const int col = blockIdx.x * 16 + (lane & 15); /* B column / C column of this lane */
-    const int g = lane >> 4;                       /* lane group: k sub-range and C rows */
-    const int nK = (N + KB - 1) / KB;              /* k-blocks; k >= N reads zero padding */
-    for (int yBase = w * kRowsPerWave; yBase < m; yBase += kWavesPerBlock * kRowsPerWave) {
-        f32x4 acc[4];
+    const int c = lane & 15, g = lane >> 4;
+    const int cb = blockIdx.x;
+    const int nK = K / 32;      /* k-blocks of 32; K is a multiple of 256 */
+    const int nS = nK / kDepth; /* super-blocks */
+    const int sOff = (int)(((long long)cb * 3) % nS);
+    const uint4 *fb = frag + (size_t)cb * nK * 2 * kWave + lane;
+    for (int y0 = 0; y0 < m; y0 += kRowsPerPass) {
+        f32x4 acc[kTiles];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < kTiles; ++t)
             acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const long long rowOff = (long long)col * Np;
-        i16x8 bh[2][kDepth], bl[2][kDepth];
-        uint32_t qb[2][kDepth][4];
-        auto fetch = [&](int buf, int kb0) {
+        uint4 bh[2][kDepth], bl[2][kDepth], qw[2][kTiles / 4];
+        /* loads super-block l (logical index) into buffer buf; l < nS */
+        auto fetch = [&](int buf, int l) {
+            int s = l + sOff;
+            if (s >= nS)
+                s -= nS;
+            const int kb0 = s * kDepth;
 #pragma unroll
             for (int d = 0; d < kDepth; ++d) {
-                const int kk = (kb0 + d) * KB + 8 * g; /* this lane's 8 k values */
-                const bool live = (kb0 + d) < nK;
-                /* J has N rows: padding columns col >= N contribute zero */
-                if (live && col < N) {
-                    bh[buf][d] = *reinterpret_cast<const i16x8 *>(Jhi + rowOff + kk);
-                    bl[buf][d] = *reinterpret_cast<const i16x8 *>(Jlo + rowOff + kk);
-                } else {
-                    bh[buf][d] = i16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                    bl[buf][d] = bh[buf][d];
-                }
+                const uint4 *p = fb + (size_t)(kb0 + d) * 2 * kWave;
+                bh[buf][d] = p[0];
+                bl[buf][d] = p[kWave];
+            }
+            /* words kb0 .. kb0+3 of row y0 + 64 r + lane (kb0 is a multiple of 4, Nw of 8) */
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    qb[buf][d][t] = live ? spinBits(Qbits, yBase + 16 * t + (lane & 15), m, Nw, kk, 0xffu) : 0u;
+            for (int r = 0; r < kTiles / 4; ++r) {
+                const int y = y0 + 64 * r + lane;
+                qw[buf][r] = (y < m) ? *reinterpret_cast<const uint4 *>(Qbits + (size_t)y * Nw + kb0)
+                                     : uint4{0u, 0u, 0u, 0u};
             }
         };
         auto compute = [&](int buf) {
@@ -91,65 +126,73 @@ __global__ __launch_bounds__(kWavesPerBlock *kWave) void fieldRefreshBF16x2Kerne
                 const bf16x8 bhi = __builtin_bit_cast(bf16x8, bh[buf][d]);
                 const bf16x8 blo = __builtin_bit_cast(bf16x8, bl[buf][d]);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const bf16x8 a = spinsBF16(qb[buf][d][t]);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bhi, acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, blo, acc[t], 0, 0, 0);
+                for (int t = 0; t < kTiles; ++t) {
+                    if (y0 + 16 * t < m) { /* wave-uniform */
+                        const uint4 q4 = qw[buf][t >> 2];
+                        const uint32_t qd = d == 0 ? q4.x : d == 1 ? q4.y : d == 2 ? q4.z : q4.w;
+                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * (t & 3) + c, kWave);
+                        const bf16x8 a = spinsBF16((word >> (8 * g)) & 0xffu);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bhi, acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, blo, acc[t], 0, 0, 0);
+                    }
                 }
             }
         };
         /* buffers alternate with constant indices (no dynamic register indexing) */
-        fetch(0, 0);
-        for (int kb0 = 0; kb0 < nK; kb0 += 2 * kDepth) {
-            if (kb0 + kDepth < nK)
-                fetch(1, kb0 + kDepth);
+        for (int l = w; l < nS; l += 2 * kSplitK) {
+            if (l == w)
+                fetch(0, l);
+            if (l + kSplitK < nS)
+                fetch(1, l + kSplitK);
             compute(0);
-            if (kb0 + kDepth >= nK)
+            if (l + kSplitK >= nS)
                 break;
-            if (kb0 + 2 * kDepth < nK)
-                fetch(0, kb0 + 2 * kDepth);
+            if (l + 2 * kSplitK < nS)
+                fetch(0, l + 2 * kSplitK);
             compute(1);
         }
-        /* C: lane holds column col, rows 4 g .. 4 g + 3 of each tile */
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < kTiles; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int y = yBase + 16 * t + 4 * g + r;
-                if (y < m)
-                    F[(long long)y * Np + col] = acc[t][r];
-            }
+            for (int r = 0; r < 4; ++r)
+                red[w][4 * t + r][lane] = acc[t][r];
+        reduceStore(red, y0, m, cb, ldF, F);
     }
 }
 
-__global__ __launch_bounds__(kWavesPerBlock *kWave) void fieldRefreshF32Kernel(const float *J, const uint32_t *Qbits,
-                                                                               int N, int Np, int m, int Nw,
-                                                                               float *F) {
-    constexpr int KB = 16; /* k per block: four 16x16x4 MFMAs */
-    const int lane = threadIdx.x & (kWave - 1);
+/* frag: [column blocks][K/16 k-blocks][64 lanes] x float4 */
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshF32Kernel(const float4 *frag, const uint32_t *Qbits,
+                                                                       int K, int m, int Nw, int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave];
+    const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
-    const int col = blockIdx.x * 16 + (lane & 15);
-    const int g = lane >> 4;
-    const int nK = (N + KB - 1) / KB;
-    for (int yBase = w * kRowsPerWave; yBase < m; yBase += kWavesPerBlock * kRowsPerWave) {
-        f32x4 acc[4];
+    const int c = lane & 15, g = lane >> 4;
+    const int cb = blockIdx.x;
+    const int nK = K / 16; /* k-blocks of 16 (four 16x16x4 steps) */
+    const int nS = nK / kDepth;
+    const int sOff = (int)(((long long)cb * 3) % nS);
+    const float4 *fb = frag + (size_t)cb * nK * kWave + lane;
+    for (int y0 = 0; y0 < m; y0 += kRowsPerPass) {
+        f32x4 acc[kTiles];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < kTiles; ++t)
             acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const long long rowOff = (long long)col * Np;
         float4 bv[2][kDepth];
-        uint32_t qb[2][kDepth][4];
-        auto fetch = [&](int buf, int kb0) {
+        uint2 qw[2][kTiles / 4];
+        auto fetch = [&](int buf, int l) {
+            int s = l + sOff;
+            if (s >= nS)
+                s -= nS;
+            const int kb0 = s * kDepth;
 #pragma unroll
-            for (int d = 0; d < kDepth; ++d) {
-                /* lane group g holds k = kk + s for MFMA step s (same for A and B) */
-                const int kk = (kb0 + d) * KB + 4 * g;
-                const bool live = (kb0 + d) < nK;
-                bv[buf][d] = (live && col < N) ? *reinterpret_cast<const float4 *>(J + rowOff + kk)
-                                               : float4{0.f, 0.f, 0.f, 0.f};
+            for (int d = 0; d < kDepth; ++d)
+                bv[buf][d] = fb[(size_t)(kb0 + d) * kWave];
+            /* a super-block spans 64 k: words kb0/2, kb0/2 + 1 (kb0 a multiple of 4) */
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    qb[buf][d][t] = live ? spinBits(Qbits, yBase + 16 * t + (lane & 15), m, Nw, kk, 0xfu) : 0u;
+            for (int r = 0; r < kTiles / 4; ++r) {
+                const int y = y0 + 64 * r + lane;
+                qw[buf][r] = (y < m) ? *reinterpret_cast<const uint2 *>(Qbits + (size_t)y * Nw + kb0 / 2)
+                                     : uint2{0u, 0u};
             }
         };
         auto compute = [&](int buf) {
@@ -157,49 +200,137 @@ __global__ __launch_bounds__(kWavesPerBlock *kWave) void fieldRefreshF32Kernel(c
             for (int d = 0; d < kDepth; ++d) {
                 const float b[4] = {bv[buf][d].x, bv[buf][d].y, bv[buf][d].z, bv[buf][d].w};
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int y = yBase + 16 * t + (lane & 15);
+                for (int t = 0; t < kTiles; ++t) {
+                    if (y0 + 16 * t < m) {
+                        const uint2 q2 = qw[buf][t >> 2];
+                        const uint32_t qd = (d >> 1) ? q2.y : q2.x;
+                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * (t & 3) + c, kWave);
+                        /* lane group g holds k = 16 kb + 4 g + s for step s (same for A and B) */
+                        const uint32_t nib = word >> (16 * (d & 1) + 4 * g);
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const float a = (y < m) ? (((qb[buf][d][t] >> s) & 1u) ? 1.f : -1.f) : 0.f;
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc[t], 0, 0, 0);
+                        for (int s = 0; s < 4; ++s) {
+                            const float a = ((nib >> s) & 1u) ? 1.f : -1.f;
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc[t], 0, 0, 0);
+                        }
                     }
                 }
             }
         };
-        fetch(0, 0);
-        for (int kb0 = 0; kb0 < nK; kb0 += 2 * kDepth) {
-            if (kb0 + kDepth < nK)
-                fetch(1, kb0 + kDepth);
+        for (int l = w; l < nS; l += 2 * kSplitK) {
+            if (l == w)
+                fetch(0, l);
+            if (l + kSplitK < nS)
+                fetch(1, l + kSplitK);
             compute(0);
-            if (kb0 + kDepth >= nK)
+            if (l + kSplitK >= nS)
                 break;
-            if (kb0 + 2 * kDepth < nK)
-                fetch(0, kb0 + 2 * kDepth);
+            if (l + 2 * kSplitK < nS)
+                fetch(0, l + 2 * kSplitK);
             compute(1);
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < kTiles; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int y = yBase + 16 * t + 4 * g + r;
-                if (y < m)
-                    F[(long long)y * Np + col] = acc[t][r];
-            }
+            for (int r = 0; r < 4; ++r)
+                red[w][4 * t + r][lane] = acc[t][r];
+        reduceStore(red, y0, m, cb, ldF, F);
     }
+}
+
+/* One thread per (column block, k-block, lane): the lane's 8 k values of its
+ * column (a row of B, row stride ld, zero past `rows`), split into bf16 hi and lo. */
+__global__ void buildFragBF16x2Kernel(const float *B, int rows, int ld, int nColBlocks, int K, uint4 *frag) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nK = K / 32;
+    if (idx >= (size_t)nColBlocks * nK * kWave)
+        return;
+    const int lane = (int)(idx % kWave);
+    const size_t rest = idx / kWave;
+    const int kb = (int)(rest % nK), cb = (int)(rest / nK);
+    const int col = cb * 16 + (lane & 15), k0 = kb * 32 + 8 * (lane >> 4);
+    uint32_t hw[4] = {0u, 0u, 0u, 0u}, lw[4] = {0u, 0u, 0u, 0u};
+    if (col < rows) {
+        const float4 *src = reinterpret_cast<const float4 *>(B + (size_t)col * ld + k0);
+        const float4 v0 = src[0], v1 = src[1];
+        const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            uint16_t hi, lo;
+            bf16Split(v[e], hi, lo);
+            hw[e >> 1] |= (uint32_t)hi << (16 * (e & 1));
+            lw[e >> 1] |= (uint32_t)lo << (16 * (e & 1));
+        }
+    }
+    uint4 *dst = frag + ((size_t)cb * nK + kb) * 2 * kWave + lane;
+    dst[0] = uint4{hw[0], hw[1], hw[2], hw[3]};
+    dst[kWave] = uint4{lw[0], lw[1], lw[2], lw[3]};
+}
+
+__global__ void buildFragF32Kernel(const float *B, int rows, int ld, int nColBlocks, int K, float4 *frag) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nK = K / 16;
+    if (idx >= (size_t)nColBlocks * nK * kWave)
+        return;
+    const int lane = (int)(idx % kWave);
+    const size_t rest = idx / kWave;
+    const int kb = (int)(rest % nK), cb = (int)(rest / nK);
+    const int col = cb * 16 + (lane & 15), k0 = kb * 16 + 4 * (lane >> 4);
+    frag[idx] = (col < rows) ? *reinterpret_cast<const float4 *>(B + (size_t)col * ld + k0) : float4{0.f, 0.f, 0.f, 0.f};
+}
+
+void checkGemm(int nCols, int K, int ld, int Nw) {
+    SQ_THROW_IF(nCols <= 0 || nCols % 16 != 0, "field GEMM: %d output columns is not a positive multiple of 16",
+                nCols);
+    SQ_THROW_IF(K <= 0 || K % 256 != 0 || Nw * 32 != K, "field GEMM: K=%d must be a positive multiple of 256", K);
+    SQ_THROW_IF(ld < K, "field GEMM: row stride %d is shorter than K=%d", ld, K);
 }
 
 }  // namespace
 
-void fieldRefreshBF16x2(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np, int m,
-                        int Nw, float *d_F, hipStream_t stream) {
-    fieldRefreshBF16x2Kernel<<<Np / 16, kWavesPerBlock * kWave, 0, stream>>>(d_hi, d_lo, d_Qbits, N, Np, m, Nw, d_F);
+size_t fieldFragElemsBF16x2(int nCols, int K) { return 2 * (size_t)nCols * K; } /* hi and lo planes, uint16 each */
+size_t fieldFragElemsF32(int nCols, int K) { return (size_t)nCols * K; }
+
+void buildFragBF16x2(const float *d_B, int rows, int ld, int nCols, int K, uint16_t *d_frag, size_t fragElems,
+                     hipStream_t stream) {
+    checkGemm(nCols, K, ld, K / 32);
+    SQ_THROW_IF(rows > nCols, "field GEMM: %d rows exceed %d columns", rows, nCols);
+    SQ_THROW_IF(fragElems < fieldFragElemsBF16x2(nCols, K), "bf16x2 fragment buffer holds %zu of %zu elements",
+                fragElems, fieldFragElemsBF16x2(nCols, K));
+    const size_t threads = (size_t)(nCols / 16) * (K / 32) * kWave;
+    buildFragBF16x2Kernel<<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(
+        d_B, rows, ld, nCols / 16, K, reinterpret_cast<uint4 *>(d_frag));
     HIP_CHECK(hipGetLastError());
 }
 
-void fieldRefreshF32MFMA(const float *d_J, const uint32_t *d_Qbits, int N, int Np, int m, int Nw, float *d_F,
-                         hipStream_t stream) {
-    fieldRefreshF32Kernel<<<Np / 16, kWavesPerBlock * kWave, 0, stream>>>(d_J, d_Qbits, N, Np, m, Nw, d_F);
+void buildFragF32(const float *d_B, int rows, int ld, int nCols, int K, float *d_frag, size_t fragElems,
+                  hipStream_t stream) {
+    checkGemm(nCols, K, ld, K / 32);
+    SQ_THROW_IF(rows > nCols, "field GEMM: %d rows exceed %d columns", rows, nCols);
+    SQ_THROW_IF(fragElems < fieldFragElemsF32(nCols, K), "fp32 fragment buffer holds %zu of %zu elements", fragElems,
+                fieldFragElemsF32(nCols, K));
+    const size_t threads = (size_t)(nCols / 16) * (K / 16) * kWave;
+    buildFragF32Kernel<<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(
+        d_B, rows, ld, nCols / 16, K, reinterpret_cast<float4 *>(d_frag));
+    HIP_CHECK(hipGetLastError());
+}
+
+void fieldRefreshBF16x2(const uint16_t *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
+                        int Nw, int ldF, float *d_F, hipStream_t stream) {
+    checkGemm(nCols, K, K, Nw);
+    SQ_THROW_IF(ldF < nCols, "field GEMM: output stride %d < %d columns", ldF, nCols);
+    SQ_THROW_IF(fragElems < fieldFragElemsBF16x2(nCols, K), "bf16x2 fragment buffer too small");
+    fieldRefreshBF16x2Kernel<<<nCols / 16, kBlockThreads, 0, stream>>>(reinterpret_cast<const uint4 *>(d_frag),
+                                                                       d_Qbits, K, m, Nw, ldF, d_F);
+    HIP_CHECK(hipGetLastError());
+}
+
+void fieldRefreshF32MFMA(const float *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
+                         int Nw, int ldF, float *d_F, hipStream_t stream) {
+    checkGemm(nCols, K, K, Nw);
+    SQ_THROW_IF(ldF < nCols, "field GEMM: output stride %d < %d columns", ldF, nCols);
+    SQ_THROW_IF(fragElems < fieldFragElemsF32(nCols, K), "fp32 fragment buffer too small");
+    fieldRefreshF32Kernel<<<nCols / 16, kBlockThreads, 0, stream>>>(reinterpret_cast<const float4 *>(d_frag),
+                                                                    d_Qbits, K, m, Nw, ldF, d_F);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -1,9 +1,13 @@
 /*
- * FP64 energy recompute for the dense annealer:
- *   E_y = -c - sum_i q_yi (h_i + sum_j J_ij q_yj)
- * (DGFuncs::calculate_E, sqaodc/cpu/SharedFormulas.cpp:69-83), evaluated in
- * double from the stored real J and h, with a fixed reduction order so the
- * result does not depend on scheduling.  Also the Trotter coupling
+ * FP64 energy recompute for the annealers, evaluated in double from the
+ * stored real J and h, with a fixed reduction order so the result does not
+ * depend on scheduling:
+ *   dense     E_y = -c - sum_i q_yi (h_i + sum_j J_ij q_yj)
+ *             (DGFuncs::calculate_E, sqaodc/cpu/SharedFormulas.cpp:69-83);
+ *   bipartite E_y = -c - h0.q0_y - sum_i q1_yi (h1_i + sum_j J_ij q0_yj)
+ *             (BGFuncs::calculate_E, SharedFormulas.cpp:184-202).
+ * Both are sums over "outer" spins of (h + B^T . inner spins): the partial
+ * kernel takes the two spin sets separately.  Also the Trotter coupling
  * sum_y q_y . q_(y+1)%m of getSystemE (CPUDenseGraphAnnealer.cpp:236-240).
  */
 #include "common.h"
@@ -16,9 +20,12 @@ namespace {
 constexpr int kCols = 256;
 constexpr int kYTile = 16;
 
+/* partial[y][bx] = sum over outer spins i in block bx's 256 columns (i < nOut)
+ * of qo_yi (h_i + sum_{j<K} B[j][i] qi_yj); B row-major with stride ldB. */
 template <class real>
-__global__ __launch_bounds__(kCols) void energyPartialKernel(const real *J, const real *h, const uint32_t *Qbits,
-                                                             int N, int Np, int m, int Nw, double *partial) {
+__global__ __launch_bounds__(kCols) void energyPartialKernel(const real *B, int K, int ldB, const real *h,
+                                                             const uint32_t *Qin, int NwIn, const uint32_t *Qout,
+                                                             int NwOut, int nOut, int m, double *partial) {
     /* block (bx, by): columns i in [bx*256, +256), slices [by*16, +16) */
     __shared__ uint32_t qs[kYTile];
     __shared__ double red[kYTile][kCols / 64];
@@ -28,18 +35,18 @@ __global__ __launch_bounds__(kCols) void energyPartialKernel(const real *J, cons
 #pragma unroll
     for (int t = 0; t < kYTile; ++t)
         acc[t] = 0.;
-    for (int j0 = 0; j0 < N; j0 += 32) {
+    for (int j0 = 0; j0 < K; j0 += 32) {
         if (threadIdx.x < kYTile)
-            qs[threadIdx.x] = (y0 + (int)threadIdx.x < m) ? Qbits[(long long)(y0 + threadIdx.x) * Nw + (j0 >> 5)] : 0u;
+            qs[threadIdx.x] = (y0 + (int)threadIdx.x < m) ? Qin[(long long)(y0 + threadIdx.x) * NwIn + (j0 >> 5)] : 0u;
         __syncthreads();
         uint32_t qw[kYTile];
 #pragma unroll
         for (int t = 0; t < kYTile; ++t)
             qw[t] = qs[t];
-        const int jn = (N - j0) < 32 ? (N - j0) : 32;
+        const int jn = (K - j0) < 32 ? (K - j0) : 32;
         for (int jj = 0; jj < jn; ++jj) {
-            /* J symmetric: sum_j J_ij q_j read down column i (coalesced) */
-            const double v = (double)J[(long long)(j0 + jj) * Np + i];
+            /* sum_j B_ji q_j read down column i (coalesced) */
+            const double v = (double)B[(long long)(j0 + jj) * ldB + i];
 #pragma unroll
             for (int t = 0; t < kYTile; ++t)
                 acc[t] += ((qw[t] >> jj) & 1u) ? v : -v;
@@ -47,13 +54,13 @@ __global__ __launch_bounds__(kCols) void energyPartialKernel(const real *J, cons
         __syncthreads();
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const bool colOk = i < N;
+    const bool colOk = i < nOut;
     const double hi = colOk ? (double)h[i] : 0.;
 #pragma unroll
     for (int t = 0; t < kYTile; ++t) {
         double v = 0.;
         if (colOk && y0 + t < m) {
-            const bool qp = (Qbits[(long long)(y0 + t) * Nw + (i >> 5)] >> (i & 31)) & 1u;
+            const bool qp = (Qout[(long long)(y0 + t) * NwOut + (i >> 5)] >> (i & 31)) & 1u;
             const double s = hi + acc[t];
             v = qp ? s : -s;
         }
@@ -73,14 +80,19 @@ __global__ __launch_bounds__(kCols) void energyPartialKernel(const real *J, cons
     }
 }
 
-__global__ void energyFinalKernel(const double *partial, int nPart, int m, double c, double *E) {
+/* E[y] = -c - (sum of partial0[y][:]) - (sum of partial1[y][:]) */
+__global__ void energyFinalKernel(const double *partial0, int nPart0, const double *partial1, int nPart1, int m,
+                                  double c, double *E) {
     int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= m)
         return;
     double s = 0.;
-    for (int b = 0; b < nPart; ++b)
-        s += partial[(long long)y * nPart + b];
-    E[y] = -c - s;
+    for (int b = 0; b < nPart0; ++b)
+        s += partial0[(long long)y * nPart0 + b];
+    double s1 = 0.;
+    for (int b = 0; b < nPart1; ++b)
+        s1 += partial1[(long long)y * nPart1 + b];
+    E[y] = -c - s - s1;
 }
 
 __global__ void spinDotSumKernel(const uint32_t *Qbits, int N, int m, int Nw, double *out) {
@@ -109,9 +121,28 @@ template <class real>
 void energyFP64(const real *d_J, const real *d_h, double c, const uint32_t *d_Qbits, int N, int Np, int m, int Nw,
                 double *d_partial, double *d_E, hipStream_t stream) {
     dim3 grid(Np / kCols, divru(m, kYTile));
-    energyPartialKernel<real><<<grid, kCols, 0, stream>>>(d_J, d_h, d_Qbits, N, Np, m, Nw, d_partial);
+    energyPartialKernel<real><<<grid, kCols, 0, stream>>>(d_J, N, Np, d_h, d_Qbits, Nw, d_Qbits, Nw, N, m, d_partial);
     HIP_CHECK(hipGetLastError());
-    energyFinalKernel<<<divru(m, 64), 64, 0, stream>>>(d_partial, (int)grid.x, m, c, d_E);
+    energyFinalKernel<<<divru(m, 64), 64, 0, stream>>>(d_partial, (int)grid.x, nullptr, 0, m, c, d_E);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <class real>
+void energyBipartiteFP64(const real *d_JT, const real *d_h0, const real *d_h1, double c, const uint32_t *d_Q0,
+                         const uint32_t *d_Q1, int N0, int N1, int N0p, int N1p, int m, double *d_partial,
+                         double *d_E, hipStream_t stream) {
+    SQ_THROW_IF(N0p % kCols != 0 || N1p % kCols != 0, "bipartite energy: padded sizes must be multiples of %d", kCols);
+    const int N0w = N0p / 32, N1w = N1p / 32;
+    /* sum_i q1_i (h1_i + sum_j J_ij q0_j) = sum_i q1_i (h1_i + sum_j JT_ji q0_j) */
+    dim3 g1(N1p / kCols, divru(m, kYTile));
+    double *p1 = d_partial, *p0 = d_partial + (size_t)m * g1.x;
+    energyPartialKernel<real><<<g1, kCols, 0, stream>>>(d_JT, N0, N1p, d_h1, d_Q0, N0w, d_Q1, N1w, N1, m, p1);
+    HIP_CHECK(hipGetLastError());
+    /* h0 . q0: the same kernel with an empty inner sum */
+    dim3 g0(N0p / kCols, divru(m, kYTile));
+    energyPartialKernel<real><<<g0, kCols, 0, stream>>>(d_JT, 0, N1p, d_h0, d_Q0, N0w, d_Q0, N0w, N0, m, p0);
+    HIP_CHECK(hipGetLastError());
+    energyFinalKernel<<<divru(m, 64), 64, 0, stream>>>(p1, (int)g1.x, p0, (int)g0.x, m, c, d_E);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -124,5 +155,9 @@ template void energyFP64<float>(const float *, const float *, double, const uint
                                 double *, hipStream_t);
 template void energyFP64<double>(const double *, const double *, double, const uint32_t *, int, int, int, int,
                                  double *, double *, hipStream_t);
+template void energyBipartiteFP64<float>(const float *, const float *, const float *, double, const uint32_t *,
+                                         const uint32_t *, int, int, int, int, int, double *, double *, hipStream_t);
+template void energyBipartiteFP64<double>(const double *, const double *, const double *, double, const uint32_t *,
+                                          const uint32_t *, int, int, int, int, int, double *, double *, hipStream_t);
 
 }  // namespace sqaod_rocm

@@ -45,10 +45,13 @@ template <class real>
 void philoxAttempts(unsigned long long seed, unsigned long long sweepIdx, int N, int m, uint32_t *d_X,
                     real *d_U, hipStream_t stream);
 
-/* randomizeSpin from a word stream: bit (y, x) = word[y*N+x] & 1
- * (CPUDenseGraphAnnealer.cpp:152-161: randInt(2) ? +1 : -1).  Qbits is
- * m x Nw words, bit x%32 of word x/32; padding bits are cleared. */
-void spinsFromWords(const uint32_t *d_words, int N, int m, int Nw, uint32_t *d_Qbits, hipStream_t stream);
+/* randomizeSpin from a word stream: bit (y, x) = word[y*rowStride+x] & 1
+ * (CPUDenseGraphAnnealer.cpp:152-161: randInt(2) ? +1 : -1; the bipartite
+ * annealer interleaves the q0 and q1 rows of each slice,
+ * CPUBipartiteGraphAnnealer.cpp:183-192).  Qbits is m x Nw words, bit x%32 of
+ * word x/32; padding bits are cleared. */
+void spinsFromWords(const uint32_t *d_words, long long rowStride, int N, int m, int Nw, uint32_t *d_Qbits,
+                    hipStream_t stream);
 /* randomizeSpin from Philox (domain separated from the attempt streams). */
 void spinsFromPhilox(unsigned long long seed, unsigned long long salt, int N, int m, int Nw, uint32_t *d_Qbits,
                      hipStream_t stream);
@@ -64,24 +67,90 @@ enum FieldFormat {
     fieldBF16x2 = 1 /* J ~ hi + lo, both bf16, hi = bf16(J), lo = bf16(J - hi) */
 };
 
-/* Splits J (N x Np, float) into bf16 hi/lo planes of the same shape, and
- * writes Jsplit = float(hi) + float(lo) (exact in fp32): the J the sweep's
- * row updates use, so that fields stay consistent with the split GEMM. */
-void splitBF16(const float *d_J, size_t count, uint16_t *d_hi, uint16_t *d_lo, float *d_Jsplit, hipStream_t stream);
+/* Jsplit = float(bf16(J)) + float(bf16(J - bf16(J))) elementwise (exact in
+ * fp32): the J the bf16x2 sweep's row updates use, so that incremental fields
+ * stay consistent with the split GEMM. */
+void splitBF16(const float *d_J, size_t count, float *d_Jsplit, hipStream_t stream);
 
-/* F[y][i] = sum_j q[y][j] * J[j][i] for all y < m, i < Np (J symmetric, N x Np
- * with zero padding), spins from packed bits.  real = float | double. */
+/* F[y][i] = sum_{j<K} q[y][j] * B[j][i] for y < m, i < nCols (B row-major with
+ * row stride ldB >= nCols, F row stride ldF), spins from packed bits (Nw words
+ * per row), vector ALU; nCols a multiple of 256.  real = float | double.
+ * Dense annealer: B = J (symmetric, N x Np), K = N, nCols = ldB = ldF = Np. */
 template <class real>
-void fieldRefresh(const real *d_J, const uint32_t *d_Qbits, int N, int Np, int m, int Nw, real *d_F,
+void fieldRefresh(const real *d_B, int K, int nCols, int ldB, const uint32_t *d_Qbits, int m, int Nw, int ldF,
+                  real *d_F, hipStream_t stream);
+
+/* MFMA forms (dense_field_mfma.hip): F[y][c] = sum_k q[y][k] B[c][k] for
+ * c < nCols (a multiple of 16), k < K (a multiple of 256, Nw = K/32 spin words
+ * per row), fp32 accumulation, F row stride ldF.  B (rows x ld floats, zero
+ * past its valid k) is first copied into MFMA fragment order:
+ * fieldFragElemsBF16x2 = 2 nCols K uint16 (hi and lo planes) or
+ * fieldFragElemsF32 = nCols K floats.  The launchers check the buffer sizes
+ * they are given.  Dense annealer: B = J, rows = N, ld = nCols = K = Np. */
+size_t fieldFragElemsBF16x2(int nCols, int K);
+size_t fieldFragElemsF32(int nCols, int K);
+void buildFragBF16x2(const float *d_B, int rows, int ld, int nCols, int K, uint16_t *d_frag, size_t fragElems,
+                     hipStream_t stream);
+void buildFragF32(const float *d_B, int rows, int ld, int nCols, int K, float *d_frag, size_t fragElems,
                   hipStream_t stream);
-/* MFMA forms (dense_field_mfma.hip): J given as bf16 hi + lo planes, or fp32
- * J on the fp32 MFMA; fp32 accumulation.  J has N rows, Np columns. */
-void fieldRefreshBF16x2Vector(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np,
-                              int m, int Nw, float *d_F, hipStream_t stream);
-void fieldRefreshF32MFMA(const float *d_J, const uint32_t *d_Qbits, int N, int Np, int m, int Nw, float *d_F,
-                         hipStream_t stream);
-void fieldRefreshBF16x2(const uint16_t *d_hi, const uint16_t *d_lo, const uint32_t *d_Qbits, int N, int Np, int m,
-                        int Nw, float *d_F, hipStream_t stream);
+void fieldRefreshBF16x2(const uint16_t *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
+                        int Nw, int ldF, float *d_F, hipStream_t stream);
+void fieldRefreshF32MFMA(const float *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
+                         int Nw, int ldF, float *d_F, hipStream_t stream);
+
+/* ---------------- bipartite annealer flips (bipartite.hip) ---------------- */
+
+/* One flip phase of a bipartite half-step: slices slice0, slice0 + sliceStep,
+ * ... of the annealed part (N spins per slice in Nw words, Nw even), with the
+ * half-step's dEmat (m x ldE).  Coloring (sa = 0): dE = twoDivM q (h + dEmat)
+ * - q (q_prev + q_next) coef, uniforms at serial position rank(im) * N + iq
+ * with rank = im / 2 for even im, nEven + im / 2 for odd.  SA (sa = 1):
+ * dE = 2 q (h + dEmat), beta = 1/kT, rank = im.  Uniforms from `words` (the
+ * half-step's MT19937 words, 1 per float uniform, 2 per double) or, when
+ * words is null, from Philox keyed (key, ctr0, ctr1) at the same position. */
+template <class real>
+struct BGFlipArgs {
+    const real *dEmat;
+    int ldE;
+    const real *h;
+    uint32_t *Q;
+    int N, Nw, m;
+    int slice0, sliceStep, nEven, sa;
+    real twoDivM, coef, beta;
+    const uint32_t *words;
+    uint32_t key0, key1, ctr0, ctr1;
+};
+template <class real>
+void bipartiteFlipPhase(const BGFlipArgs<real> &a, int nSlices, hipStream_t stream);
+
+/* ---------------- dense brute-force search (dense_bf.hip) ---------------- */
+
+constexpr int kBFMaxLowBits = 12;      /* low bits of x enumerated per block (2^12-entry LDS table) */
+constexpr int kBFMaxHighPatterns = 16; /* high patterns per block */
+
+/* Split of x into high and low bits for N (<= 63) */
+struct BFPlan {
+    int N, L, nH, Llo, lgR, R;
+};
+BFPlan bfPlan(int N);
+
+/* Device-resident search state: the minimum so far and the list length. */
+template <class real>
+struct BFState {
+    real Emin;
+    unsigned long long count;
+};
+
+/* Scans x in [xb, xe) (E = x^T W x with W: N x N double, EL: the 2^L-entry
+ * low-bit energy table) and merges the range into *d_state and d_xList
+ * (capacity cap) with the reference's list rule; asynchronous, no host
+ * read-back.  The scratch arrays hold maxBlocks entries; the range may span at
+ * most maxBlocks units of 2^(L + lgR) candidates. */
+template <class real>
+void bfSearchRange(const double *d_W, const double *d_EL, const BFPlan &p, unsigned long long xb,
+                   unsigned long long xe, BFState<real> *d_state, unsigned long long cap, real *d_blockMin,
+                   unsigned *d_blockCnt, unsigned long long *d_offsets, int maxBlocks,
+                   unsigned long long *d_xList, hipStream_t stream);
 
 /* ---------------- dense sweep (dense_sweep.hip) ---------------- */
 
@@ -121,6 +190,14 @@ size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plai
 template <class real>
 void energyFP64(const real *d_J, const real *d_h, double c, const uint32_t *d_Qbits, int N, int Np, int m, int Nw,
                 double *d_partial, double *d_E, hipStream_t stream);
+
+/* Bipartite FP64 recompute E[y] = -c - h0.q0_y - h1.q1_y - q1_y^T J q0_y
+ * from JT = J^T (N0 x N1p, zero padded), spins from packed bits (N0p/32 and
+ * N1p/32 words per row); d_partial holds m * (N0p + N1p) / 256 doubles. */
+template <class real>
+void energyBipartiteFP64(const real *d_JT, const real *d_h0, const real *d_h1, double c, const uint32_t *d_Q0,
+                         const uint32_t *d_Q1, int N0, int N1, int N0p, int N1p, int m, double *d_partial,
+                         double *d_E, hipStream_t stream);
 
 /* sum_y q_y . q_(y+1)%m (the Trotter coupling of getSystemE), in double. */
 void spinDotSum(const uint32_t *d_Qbits, int N, int m, int Nw, double *d_out, hipStream_t stream);

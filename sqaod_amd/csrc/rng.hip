@@ -16,6 +16,7 @@
  */
 #include "common.h"
 #include "kernels.h"
+#include "rng_device.h"
 
 namespace sqaod_rocm {
 
@@ -81,34 +82,6 @@ __global__ __launch_bounds__(1024) void mtGenerateKernel(MTState *st, uint32_t *
         st->idx = idx;
 }
 
-/* Philox4x32-10 */
-struct U4 {
-    uint32_t x, y, z, w;
-};
-
-__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-        k0 += W0;
-        k1 += W1;
-    }
-    return c;
-}
-
-__device__ __forceinline__ float wordToUniform(uint32_t w) {
-    return (float)w * (float)(1. / 4294967296.);
-}
-
-__device__ __forceinline__ double wordsToUniform(uint32_t w1, uint32_t w2) {
-    uint32_t a = w1 >> 5, b = w2 >> 6;
-    return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
-}
-
 template <class real>
 struct Draws;
 template <>
@@ -158,7 +131,7 @@ __global__ void philoxAttemptsKernel(uint32_t k0, uint32_t k1, uint32_t s0, uint
     U[idx] = Draws<real>::uniform(r);
 }
 
-__global__ void spinsFromWordsKernel(const uint32_t *words, int N, int m, int Nw, uint32_t *Q) {
+__global__ void spinsFromWordsKernel(const uint32_t *words, long long rowStride, int N, int m, int Nw, uint32_t *Q) {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
     if (w >= Nw || y >= m)
@@ -166,7 +139,7 @@ __global__ void spinsFromWordsKernel(const uint32_t *words, int N, int m, int Nw
     uint32_t bits = 0;
     for (int b = 0; b < 32; ++b) {
         int x = w * 32 + b;
-        if (x < N && (words[(long long)y * N + x] & 1u))
+        if (x < N && (words[(long long)y * rowStride + x] & 1u))
             bits |= 1u << b;
     }
     Q[(long long)y * Nw + w] = bits;
@@ -247,9 +220,11 @@ void philoxAttempts(unsigned long long seed, unsigned long long sweepIdx, int N,
     HIP_CHECK(hipGetLastError());
 }
 
-void spinsFromWords(const uint32_t *d_words, int N, int m, int Nw, uint32_t *d_Qbits, hipStream_t stream) {
+void spinsFromWords(const uint32_t *d_words, long long rowStride, int N, int m, int Nw, uint32_t *d_Qbits,
+                    hipStream_t stream) {
+    SQ_THROW_IF(rowStride < N, "spinsFromWords: row stride %lld < N=%d", rowStride, N);
     dim3 grid(gridFor(Nw, 64), m);
-    spinsFromWordsKernel<<<grid, 64, 0, stream>>>(d_words, N, m, Nw, d_Qbits);
+    spinsFromWordsKernel<<<grid, 64, 0, stream>>>(d_words, rowStride, N, m, Nw, d_Qbits);
     HIP_CHECK(hipGetLastError());
 }
 

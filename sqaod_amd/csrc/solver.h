@@ -186,13 +186,15 @@ private:
     void collectPhases();
 
     bool fieldsDirty_ = true;
+    bool fragValid_ = false; /* d_frag16_/d_frag32_ hold the current J in MFMA fragment order */
     int stepsSinceRefresh_ = 0;
     double lastERoundErr_ = 0.;
 
     /* device state */
     DevBuf<real> d_J_, d_h_, d_F_, d_U_;
     DevBuf<float> d_Jsplit_;  /* bf16x2 form: float(hi) + float(lo) */
-    DevBuf<uint16_t> d_Jhi_, d_Jlo_;
+    DevBuf<uint16_t> d_frag16_; /* bf16x2 MFMA refresh: J's hi/lo planes in fragment order */
+    DevBuf<float> d_frag32_;    /* fp32 MFMA refresh: J in fragment order */
     DevBuf<uint32_t> d_Q_, d_X_, d_D_, d_words_;
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
@@ -207,6 +209,132 @@ private:
     std::vector<real> h_;     /* host h (for getHamiltonian) */
     std::vector<real> E_;
     std::vector<int8_t> q_;   /* m x N after makeSolution */
+};
+
+/* ------------------------------------------------------------------ */
+/* Dense-graph brute-force searcher (CPUDenseGraphBFSearcher /           */
+/* CUDADenseGraphBFSearcher)                                             */
+/* ------------------------------------------------------------------ */
+template <class real>
+class DenseGraphBFSearcher : public SolverState {
+public:
+    using real_type = real;
+    DenseGraphBFSearcher() = default;
+    ~DenseGraphBFSearcher() = default;
+
+    void assignDevice(Device &dev);
+    void setPreference(const std::string &name, const std::string &value);
+    PrefList getPreferences() const;
+
+    void setQUBO(const real *W, int N, OptimizeMethod om);
+    int getProblemSize() const { return N_; }
+    void prepare();
+    /* Scans the next span of x (max(tile size, 2^26) candidates, clipped to
+     * 2^N) on the device without a host round trip; returns true when the
+     * whole space has been scanned.  *curXEnd receives the end of the span. */
+    bool searchRange(unsigned long long *curXEnd);
+    void search();
+    void makeSolution();
+    void calculate_E();
+    int numSolutions();           /* rows of get_x (makes the solution if needed) */
+    int numEnergies();            /* length of get_E (calculates E if needed; at least 1) */
+    void get_x(int8_t *x);        /* numSolutions() x N */
+    void get_E(real *E);          /* numEnergies() */
+
+private:
+    void ensureDevice();
+
+    Device *dev_ = nullptr;
+    int N_ = 0;
+    OptimizeMethod om_ = optMinimize;
+    int tileSize_ = 1024; /* CPUDenseGraphBFSearcher.cpp:16 */
+    std::vector<double> W_; /* N x N, negated for maximize (:37-38) */
+    BFPlan plan_{};
+    unsigned long long x_ = 0, xMax_ = 0, span_ = 0;
+    int maxBlocks_ = 0;
+    real Emin_ = 0;
+    std::vector<unsigned long long> xList_;
+    std::vector<real> E_;
+
+    DevBuf<double> d_W_, d_EL_;
+    DevBuf<BFState<real>> d_state_;
+    DevBuf<real> d_blockMin_;
+    DevBuf<unsigned> d_blockCnt_;
+    DevBuf<unsigned long long> d_offsets_, d_xList_;
+};
+
+/* ------------------------------------------------------------------ */
+/* Bipartite-graph annealer (CPUBipartiteGraphAnnealer /                 */
+/* CUDABipartiteGraphAnnealer)                                           */
+/* ------------------------------------------------------------------ */
+template <class real>
+class BipartiteGraphAnnealer : public SolverState {
+public:
+    using real_type = real;
+    BipartiteGraphAnnealer() = default;
+    ~BipartiteGraphAnnealer() = default;
+
+    void assignDevice(Device &dev);
+    void seed(unsigned long long s);
+    Algorithm selectAlgorithm(Algorithm algo);
+    Algorithm getAlgorithm() const { return algo_; }
+    void setPreference(const std::string &name, const std::string &value);
+    PrefList getPreferences() const;
+
+    /* W is N1 x N0 (rows: the x1 side), b0[N0], b1[N1] */
+    void setQUBO(const real *b0, const real *b1, const real *W, int N0, int N1, OptimizeMethod om);
+    void setHamiltonian(const real *h0, const real *h1, const real *J, real c, int N0, int N1);
+    void getHamiltonian(real *h0, real *h1, real *J, real *c) const;
+    void getProblemSize(int *N0, int *N1) const;
+    int numTrotters() const { return m_; }
+
+    void set_q(const int8_t *q0, int N0, const int8_t *q1, int N1); /* one pair for every slice */
+    void set_qset(const int8_t *q0, const int8_t *q1, int m, int N0, int N1);
+    void get_q(int8_t *q0, int8_t *q1); /* m x N0, m x N1 */
+    void get_x(int8_t *x0, int8_t *x1);
+    void get_E(real *E);
+    void randomizeSpin();
+    void prepare();
+    void calculate_E();
+    void makeSolution();
+    real getSystemE(real G, real beta);
+    void annealOneStep(real G, real beta);
+
+private:
+    void ensureDevice();
+    void uploadProblem(const std::vector<double> &h0, const std::vector<double> &h1, const std::vector<double> &J,
+                       double c);
+    /* dEmat of one half-step: side 1 anneals q1 against q0 (dE = Q0 J^T),
+     * side 0 anneals q0 against q1 (dE = Q1 J) */
+    void computeFields(int side);
+    void halfStep(int side, real G, real beta, const uint32_t *words);
+
+    Device *dev_ = nullptr;
+    int N0_ = 0, N1_ = 0, N0p_ = 0, N1p_ = 0, m_ = -1;
+    Algorithm algo_ = algoColoring;
+    OptimizeMethod om_ = optMinimize;
+    unsigned long long seed_ = 0;
+    unsigned long long stepIdx_ = 0;
+    real c_ = 0;
+
+    /* preferences beyond the reference's */
+    bool rngMT_ = false;         /* "rng": "philox" (default) | "mt19937" */
+    int fieldFormat_ = fieldF32; /* "field_precision": "fp32" | "bf16x2" (float solvers) */
+    bool refreshVector_ = false; /* "refresh_engine": "mfma" | "vector" */
+    bool fragValid_ = false;
+    bool mtValid_ = false;       /* d_mt_ holds the generator seeded by the last seed() */
+
+    std::vector<real> h0_, h1_;
+    std::vector<real> E_;
+    std::vector<int8_t> q0_, q1_;
+
+    DevBuf<real> d_J_, d_JT_, d_h0_, d_h1_, d_dE_;
+    DevBuf<uint16_t> d_fragJ16_, d_fragJT16_;
+    DevBuf<float> d_fragJ32_, d_fragJT32_;
+    DevBuf<uint32_t> d_Q0_, d_Q1_, d_words_;
+    DevBuf<MTState> d_mt_;
+    DevBuf<double> d_E_, d_partial_, d_spinDot_;
+    DevBuf<int8_t> d_q8_;
 };
 
 }  // namespace sqaod_rocm

@@ -56,6 +56,43 @@ _SIGS = {
     "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
     "sqaod_rocm_dg_annealer_get_fields": (_i32, [_vp, _vp, _i32, _i32]),
     "sqaod_rocm_dg_annealer_get_phase_times": (_i32, [_vp, ctypes.POINTER(_f64), ctypes.POINTER(ctypes.c_ulonglong)]),
+    "sqaod_rocm_dg_bf_searcher_new": (_i32, [_i32, _PP]),
+    "sqaod_rocm_dg_bf_searcher_delete": (_i32, [_vp]),
+    "sqaod_rocm_dg_bf_searcher_assign_device": (_i32, [_vp, _vp]),
+    "sqaod_rocm_dg_bf_searcher_set_qubo": (_i32, [_vp, _vp, _i32, _i32]),
+    "sqaod_rocm_dg_bf_searcher_get_problem_size": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dg_bf_searcher_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "sqaod_rocm_dg_bf_searcher_get_preferences": (_i32, [_vp, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
+    "sqaod_rocm_dg_bf_searcher_prepare": (_i32, [_vp]),
+    "sqaod_rocm_dg_bf_searcher_calculate_E": (_i32, [_vp]),
+    "sqaod_rocm_dg_bf_searcher_make_solution": (_i32, [_vp]),
+    "sqaod_rocm_dg_bf_searcher_search_range": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_u64)]),
+    "sqaod_rocm_dg_bf_searcher_search": (_i32, [_vp]),
+    "sqaod_rocm_dg_bf_searcher_get_n_solutions": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dg_bf_searcher_get_x": (_i32, [_vp, _vp, _i32, _i32]),
+    "sqaod_rocm_dg_bf_searcher_get_E": (_i32, [_vp, _vp, _i32]),
+    "sqaod_rocm_bg_annealer_new": (_i32, [_i32, _PP]),
+    "sqaod_rocm_bg_annealer_delete": (_i32, [_vp]),
+    "sqaod_rocm_bg_annealer_assign_device": (_i32, [_vp, _vp]),
+    "sqaod_rocm_bg_annealer_seed": (_i32, [_vp, _u64]),
+    "sqaod_rocm_bg_annealer_set_qubo": (_i32, [_vp, _vp, _i32, _vp, _i32, _vp, _i32]),
+    "sqaod_rocm_bg_annealer_set_hamiltonian": (_i32, [_vp, _vp, _i32, _vp, _i32, _vp, _vp]),
+    "sqaod_rocm_bg_annealer_get_problem_size": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaod_rocm_bg_annealer_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "sqaod_rocm_bg_annealer_get_preferences": (_i32, [_vp, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
+    "sqaod_rocm_bg_annealer_get_n_trotters": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_bg_annealer_get_E": (_i32, [_vp, _vp, _i32]),
+    "sqaod_rocm_bg_annealer_get_x": (_i32, [_vp, _vp, _vp, _i32, _i32, _i32]),
+    "sqaod_rocm_bg_annealer_get_hamiltonian": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "sqaod_rocm_bg_annealer_get_q": (_i32, [_vp, _vp, _vp, _i32, _i32, _i32]),
+    "sqaod_rocm_bg_annealer_set_q": (_i32, [_vp, _vp, _i32, _vp, _i32]),
+    "sqaod_rocm_bg_annealer_set_qset": (_i32, [_vp, _vp, _vp, _i32, _i32, _i32]),
+    "sqaod_rocm_bg_annealer_randomize_spin": (_i32, [_vp]),
+    "sqaod_rocm_bg_annealer_calculate_E": (_i32, [_vp]),
+    "sqaod_rocm_bg_annealer_prepare": (_i32, [_vp]),
+    "sqaod_rocm_bg_annealer_make_solution": (_i32, [_vp]),
+    "sqaod_rocm_bg_annealer_get_system_E": (_i32, [_vp, _f64, _f64, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_bg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
     "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -82,6 +119,30 @@ def _dt(dtype):
     if dt == np.float32:
         return 0
     raise RuntimeError("dtype must be np.float32 or np.float64")
+
+
+_INT_PREFS = ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval', 'profile')
+
+
+def _set_prefs(setter, cobj, prefs):
+    for k, v in (prefs or {}).items():
+        if hasattr(v, '__int__') and not isinstance(v, (str, float)):
+            v = int(v)
+        _check(setter(cobj, str(k).encode(), str(v).encode()))
+
+
+def _get_prefs(getter, cobj):
+    need = _sz()
+    _check(getter(cobj, None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    _check(getter(cobj, buf, need.value, ctypes.byref(need)))
+    out = {}
+    for item in buf.value.decode().split(';'):
+        if not item:
+            continue
+        k, v = item.split('=', 1)
+        out[k] = int(v) if k in _INT_PREFS else v
+    return out
 
 
 def version():
@@ -166,25 +227,11 @@ class _DenseGraphAnnealer:
 
     @staticmethod
     def set_preferences(cobj, prefs, dtype):
-        for k, v in (prefs or {}).items():
-            if hasattr(v, '__int__') and not isinstance(v, (str, float)):
-                v = int(v)
-            _check(_lib.sqaod_rocm_dg_annealer_set_preference(cobj, str(k).encode(), str(v).encode()))
+        _set_prefs(_lib.sqaod_rocm_dg_annealer_set_preference, cobj, prefs)
 
     @staticmethod
     def get_preferences(cobj, dtype):
-        need = _sz()
-        _check(_lib.sqaod_rocm_dg_annealer_get_preferences(cobj, None, 0, ctypes.byref(need)))
-        buf = ctypes.create_string_buffer(need.value)
-        _check(_lib.sqaod_rocm_dg_annealer_get_preferences(cobj, buf, need.value, ctypes.byref(need)))
-        out = {}
-        for item in buf.value.decode().split(';'):
-            if not item:
-                continue
-            k, v = item.split('=', 1)
-            out[k] = int(v) if k in ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval',
-                                     'profile') else v
-        return out
+        return _get_prefs(_lib.sqaod_rocm_dg_annealer_get_preferences, cobj)
 
     @staticmethod
     def _m(cobj):
@@ -280,5 +327,214 @@ class _DenseGraphAnnealer:
         return e.value
 
 
+class _DenseGraphBFSearcher:
+    """rocm_dg_bf_searcher module: bf_searcher.inc:479-494 method table."""
+
+    @staticmethod
+    def new(dtype):
+        out = _vp()
+        _check(_lib.sqaod_rocm_dg_bf_searcher_new(_dt(dtype), ctypes.byref(out)))
+        return out.value
+
+    @staticmethod
+    def delete(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_bf_searcher_delete(cobj))
+
+    @staticmethod
+    def assign_device(cobj, devobj, dtype):
+        _check(_lib.sqaod_rocm_dg_bf_searcher_assign_device(cobj, devobj))
+
+    @staticmethod
+    def set_qubo(cobj, W, optimize, dtype):
+        W = np.ascontiguousarray(W, dtype)
+        if W.ndim != 2 or W.shape[0] != W.shape[1]:
+            raise RuntimeError("W must be a square matrix.")
+        _check(_lib.sqaod_rocm_dg_bf_searcher_set_qubo(cobj, _ptr(W), W.shape[0], int(optimize)))
+
+    @staticmethod
+    def get_problem_size(cobj, dtype):
+        n = _i32()
+        _check(_lib.sqaod_rocm_dg_bf_searcher_get_problem_size(cobj, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def set_preferences(cobj, prefs, dtype):
+        _set_prefs(_lib.sqaod_rocm_dg_bf_searcher_set_preference, cobj, prefs)
+
+    @staticmethod
+    def get_preferences(cobj, dtype):
+        return _get_prefs(_lib.sqaod_rocm_dg_bf_searcher_get_preferences, cobj)
+
+    @staticmethod
+    def get_x(cobj, dtype):
+        n = _i32()
+        _check(_lib.sqaod_rocm_dg_bf_searcher_get_n_solutions(cobj, ctypes.byref(n)))
+        N = _DenseGraphBFSearcher.get_problem_size(cobj, dtype)
+        x = np.empty((n.value, N), np.int8)
+        _check(_lib.sqaod_rocm_dg_bf_searcher_get_x(cobj, _ptr(x), n.value, N))
+        return [row.copy() for row in x]
+
+    @staticmethod
+    def get_E(cobj, dtype):
+        n = _i32()
+        _check(_lib.sqaod_rocm_dg_bf_searcher_get_n_solutions(cobj, ctypes.byref(n)))
+        E = np.empty(max(1, n.value), dtype)
+        _check(_lib.sqaod_rocm_dg_bf_searcher_get_E(cobj, _ptr(E), E.shape[0]))
+        return E
+
+    @staticmethod
+    def prepare(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_bf_searcher_prepare(cobj))
+
+    @staticmethod
+    def calculate_E(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_bf_searcher_calculate_E(cobj))
+
+    @staticmethod
+    def make_solution(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_bf_searcher_make_solution(cobj))
+
+    @staticmethod
+    def search_range(cobj, dtype):
+        done, cur = _i32(), _u64()
+        _check(_lib.sqaod_rocm_dg_bf_searcher_search_range(cobj, ctypes.byref(done), ctypes.byref(cur)))
+        return bool(done.value), cur.value
+
+    @staticmethod
+    def search(cobj, dtype):
+        _check(_lib.sqaod_rocm_dg_bf_searcher_search(cobj))
+
+
+class _BipartiteGraphAnnealer:
+    """rocm_bg_annealer module: annealer.inc:885-908 method table, bipartite form."""
+
+    @staticmethod
+    def new(dtype):
+        out = _vp()
+        _check(_lib.sqaod_rocm_bg_annealer_new(_dt(dtype), ctypes.byref(out)))
+        return out.value
+
+    @staticmethod
+    def delete(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_delete(cobj))
+
+    @staticmethod
+    def assign_device(cobj, devobj, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_assign_device(cobj, devobj))
+
+    @staticmethod
+    def seed(cobj, seed, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_seed(cobj, int(seed) & 0xffffffffffffffff))
+
+    @staticmethod
+    def set_qubo(cobj, b0, b1, W, optimize, dtype):
+        b0 = np.ascontiguousarray(b0, dtype).reshape(-1)
+        b1 = np.ascontiguousarray(b1, dtype).reshape(-1)
+        W = np.ascontiguousarray(W, dtype)
+        if W.shape != (b1.shape[0], b0.shape[0]):
+            raise RuntimeError("Shape mismatch: W must be N1 x N0 for b0[N0], b1[N1].")
+        _check(_lib.sqaod_rocm_bg_annealer_set_qubo(cobj, _ptr(b0), b0.shape[0], _ptr(b1), b1.shape[0], _ptr(W),
+                                                    int(optimize)))
+
+    @staticmethod
+    def set_hamiltonian(cobj, h0, h1, J, c, dtype):
+        h0 = np.ascontiguousarray(h0, dtype).reshape(-1)
+        h1 = np.ascontiguousarray(h1, dtype).reshape(-1)
+        J = np.ascontiguousarray(J, dtype)
+        cc = np.array([c], dtype)
+        if J.shape != (h1.shape[0], h0.shape[0]):
+            raise RuntimeError("Shape mismatch: J must be N1 x N0 for h0[N0], h1[N1].")
+        _check(_lib.sqaod_rocm_bg_annealer_set_hamiltonian(cobj, _ptr(h0), h0.shape[0], _ptr(h1), h1.shape[0],
+                                                           _ptr(J), _ptr(cc)))
+
+    @staticmethod
+    def get_problem_size(cobj, dtype):
+        n0, n1 = _i32(), _i32()
+        _check(_lib.sqaod_rocm_bg_annealer_get_problem_size(cobj, ctypes.byref(n0), ctypes.byref(n1)))
+        return n0.value, n1.value
+
+    @staticmethod
+    def set_preferences(cobj, prefs, dtype):
+        _set_prefs(_lib.sqaod_rocm_bg_annealer_set_preference, cobj, prefs)
+
+    @staticmethod
+    def get_preferences(cobj, dtype):
+        return _get_prefs(_lib.sqaod_rocm_bg_annealer_get_preferences, cobj)
+
+    @staticmethod
+    def _m(cobj):
+        m = _i32()
+        _check(_lib.sqaod_rocm_bg_annealer_get_n_trotters(cobj, ctypes.byref(m)))
+        return m.value
+
+    @staticmethod
+    def get_E(cobj, dtype):
+        m = _BipartiteGraphAnnealer._m(cobj)
+        E = np.empty(m, dtype)
+        _check(_lib.sqaod_rocm_bg_annealer_get_E(cobj, _ptr(E), m))
+        return E
+
+    @staticmethod
+    def _pairs(fn, cobj, dtype):
+        m = _BipartiteGraphAnnealer._m(cobj)
+        N0, N1 = _BipartiteGraphAnnealer.get_problem_size(cobj, dtype)
+        a0 = np.empty((m, N0), np.int8)
+        a1 = np.empty((m, N1), np.int8)
+        _check(fn(cobj, _ptr(a0), _ptr(a1), m, N0, N1))
+        return [(a0[i].copy(), a1[i].copy()) for i in range(m)]
+
+    @staticmethod
+    def get_x(cobj, dtype):
+        return _BipartiteGraphAnnealer._pairs(_lib.sqaod_rocm_bg_annealer_get_x, cobj, dtype)
+
+    @staticmethod
+    def get_q(cobj, dtype):
+        return _BipartiteGraphAnnealer._pairs(_lib.sqaod_rocm_bg_annealer_get_q, cobj, dtype)
+
+    @staticmethod
+    def get_hamiltonian(cobj, h0, h1, J, c, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_get_hamiltonian(cobj, _ptr(h0), _ptr(h1), _ptr(J), _ptr(c)))
+
+    @staticmethod
+    def set_q(cobj, qpair, dtype):
+        q0 = np.ascontiguousarray(qpair[0], np.int8).reshape(-1)
+        q1 = np.ascontiguousarray(qpair[1], np.int8).reshape(-1)
+        _check(_lib.sqaod_rocm_bg_annealer_set_q(cobj, _ptr(q0), q0.shape[0], _ptr(q1), q1.shape[0]))
+
+    @staticmethod
+    def set_qset(cobj, qpairs, dtype):
+        q0 = np.ascontiguousarray(np.stack([np.asarray(p[0], np.int8).reshape(-1) for p in qpairs]))
+        q1 = np.ascontiguousarray(np.stack([np.asarray(p[1], np.int8).reshape(-1) for p in qpairs]))
+        _check(_lib.sqaod_rocm_bg_annealer_set_qset(cobj, _ptr(q0), _ptr(q1), q0.shape[0], q0.shape[1], q1.shape[1]))
+
+    @staticmethod
+    def randomize_spin(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_randomize_spin(cobj))
+
+    @staticmethod
+    def calculate_E(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_calculate_E(cobj))
+
+    @staticmethod
+    def prepare(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_prepare(cobj))
+
+    @staticmethod
+    def make_solution(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_make_solution(cobj))
+
+    @staticmethod
+    def get_system_E(cobj, G, beta, dtype):
+        E = _f64()
+        _check(_lib.sqaod_rocm_bg_annealer_get_system_E(cobj, float(G), float(beta), ctypes.byref(E)))
+        return np.dtype(dtype).type(E.value)
+
+    @staticmethod
+    def anneal_one_step(cobj, G, beta, dtype):
+        _check(_lib.sqaod_rocm_bg_annealer_anneal_one_step(cobj, float(G), float(beta)))
+
+
 rocm_device = _Device
 rocm_dg_annealer = _DenseGraphAnnealer
+rocm_dg_bf_searcher = _DenseGraphBFSearcher
+rocm_bg_annealer = _BipartiteGraphAnnealer

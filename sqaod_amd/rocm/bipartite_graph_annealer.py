@@ -1,0 +1,24 @@
+"""rocm bipartite graph annealer (sqaodpy/sqaod/cuda/bipartite_graph_annealer.py)."""
+import numpy as np
+
+from ..common.bipartite_graph_annealer_base import BipartiteGraphAnnealerBase
+from ..common.preference import minimize
+from ._cext import rocm_bg_annealer as cext
+from . import device
+
+
+class BipartiteGraphAnnealer(BipartiteGraphAnnealerBase):
+    """Bipartite graph annealer on the rocm device."""
+
+    def __init__(self, b0, b1, W, optimize, dtype, prefdict):
+        self._cobj = cext.new(dtype)
+        dev = device.get_active_device()
+        cext.assign_device(self._cobj, dev._cobj, dtype)
+        self._device = dev
+        BipartiteGraphAnnealerBase.__init__(self, cext, dtype, b0, b1, W, optimize, prefdict)
+
+
+def bipartite_graph_annealer(b0=None, b1=None, W=None, optimize=minimize, dtype=np.float64, **prefs):
+    """Factory of sqaod_amd.rocm.BipartiteGraphAnnealer (same signature as
+    sqaod.cuda.bipartite_graph_annealer)."""
+    return BipartiteGraphAnnealer(b0, b1, W, optimize, dtype, prefs)

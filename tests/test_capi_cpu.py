@@ -100,9 +100,45 @@ def test_python_front_end_mirrors_reference_module():
     import sqaod_amd as sq
     import sqaod_amd.rocm as rocm
     assert sq.algorithm.coloring == "coloring" and int(sq.minimize) == 0 and int(sq.maximize) == 1
-    assert callable(rocm.dense_graph_annealer)
+    assert callable(rocm.dense_graph_annealer) and callable(rocm.dense_graph_bf_searcher)
     x = sq.create_bitset_sequence([5], 4)
     assert x.tolist() == [[0, 1, 0, 1]]
     assert sq.bit_to_spin(x).tolist() == [[-1, 1, -1, 1]]
     W = np.triu(np.ones((4, 4)))
     assert np.array_equal(sq.symmetrize(W), (W + W.T) * 0.5)
+
+
+def test_bf_searcher_host_logic_without_gpu(cext):
+    """Dense BF searcher: set_qubo checks (N <= 63, CUDADenseGraphBFSearcher.cpp:
+    54-67; symmetric W), tile_size rounding (Solver.cpp:201-209), algorithm
+    pinned to brute_force_search, and the state machine -- all before any
+    device call."""
+    B = cext.rocm_dg_bf_searcher
+    h = B.new(np.float64)
+    try:
+        with pytest.raises(RuntimeError, match="Problem is not set"):
+            B.get_problem_size(h, np.float64)
+        with pytest.raises(RuntimeError, match="Problem is not set"):
+            B.prepare(h, np.float64)
+        with pytest.raises(RuntimeError, match="smaller than 64"):
+            B.set_qubo(h, np.zeros((64, 64)), 0, np.float64)
+        with pytest.raises(RuntimeError, match="not symmetric"):
+            B.set_qubo(h, np.triu(np.ones((5, 5))), 0, np.float64)
+        B.set_qubo(h, np.ones((63, 63)), 1, np.float64)
+        assert B.get_problem_size(h, np.float64) == 63
+        with pytest.raises(RuntimeError, match="not prepared"):
+            B.search_range(h, np.float64)
+        with pytest.raises(RuntimeError, match="not prepared"):
+            B.make_solution(h, np.float64)
+        prefs = B.get_preferences(h, np.float64)
+        assert prefs == {"algorithm": "brute_force_search", "tile_size": 1024, "precision": "double",
+                         "device": "rocm"}
+        B.set_preferences(h, {"tile_size": 1000, "algorithm": "coloring"}, np.float64)
+        assert B.get_preferences(h, np.float64)["tile_size"] == 1024
+        B.set_preferences(h, {"tile_size": 1025}, np.float64)
+        assert B.get_preferences(h, np.float64)["tile_size"] == 1280
+        assert B.get_preferences(h, np.float64)["algorithm"] == "brute_force_search"
+        with pytest.raises(RuntimeError, match="positive integer"):
+            B.set_preferences(h, {"tile_size": 0}, np.float64)
+    finally:
+        B.delete(h, np.float64)

@@ -29,10 +29,11 @@ def host_fields(oracle, W, q):
 
 
 @pytest.mark.parametrize("form", ["fp32", "bf16x2"])
-@pytest.mark.parametrize("N,m", [(300, 37), (128, 32), (1000, 16)])
+@pytest.mark.parametrize("N,m", [(300, 37), (128, 32), (1000, 16), (200, 520)])
 def test_field_gemm_exact_on_integer_problems(oracle, rocm, form, N, m):
     """Quarter-integer J: every engine gives the exact product (MFMA fp32,
-    MFMA bf16 hi+lo, vector) -- including ragged N and m."""
+    MFMA bf16 hi+lo, vector) -- including ragged N and m, and m > 256 (the
+    MFMA kernels then take several passes over the slice rows)."""
     W = dense_int_W(N, SEED_BASE + N)
     res = {}
     for engine in ("mfma", "vector"):

@@ -362,3 +362,155 @@ def test_exp2_table_regenerates():
         d = float(str((ln2 * i / 32).exp()))
         u = struct.unpack("<Q", struct.pack("<d", d))[0]
         assert tab[i] == (u - (i << 47)) & 0xFFFFFFFFFFFFFFFF
+
+
+# ---------------------------------------------------------------- bipartite
+def _bg_int_problem(N0, N1, seed, lo=-3, hi=3):
+    rng = np.random.default_rng(seed)
+    b0 = rng.integers(lo, hi + 1, N0).astype(np.float64)
+    b1 = rng.integers(lo, hi + 1, N1).astype(np.float64)
+    W = rng.integers(lo, hi + 1, (N1, N0)).astype(np.float64)
+    return b0, b1, W
+
+
+def test_bipartite_hamiltonian_reproduces_qubo_energy_on_all_states(oracle):
+    """BGFuncs::calculateHamiltonian (SharedFormulas.cpp:147-167) maps the QUBO
+    E(x0, x1) = b0.x0 + b1.x1 + x1^T W x0 onto the Ising form
+    -(h0.q0 + h1.q1 + q1^T J q0) - c with q = 2x - 1 (the identity
+    test_bipartite_graph_annealer.py:98-120 checks through the annealer):
+    every (x0, x1) of a 6 x 5 problem, so a dropped term, a wrong sign or a
+    transposed W fails."""
+    N0, N1 = 6, 5
+    b0, b1, W = _bg_int_problem(N0, N1, 11)
+    h0, h1, J, c = oracle.bipartite_hamiltonian(b0, b1, W)
+    x0 = ((np.arange(1 << N0)[:, None] >> np.arange(N0 - 1, -1, -1)[None, :]) & 1).astype(np.float64)
+    x1 = ((np.arange(1 << N1)[:, None] >> np.arange(N1 - 1, -1, -1)[None, :]) & 1).astype(np.float64)
+    Eq = (x0 @ b0)[:, None] + (x1 @ b1)[None, :] + x0 @ W.T @ x1.T
+    q0, q1 = 2 * x0 - 1, 2 * x1 - 1
+    Ei = -((q0 @ h0)[:, None] + (q1 @ h1)[None, :] + q0 @ J.T @ q1.T) - c
+    assert np.array_equal(Eq, Ei)
+
+
+def test_bipartite_annealer_energy_known_answers(oracle):
+    """test_bipartite_graph_annealer.py:88-104, 164-177: q = -1 gives E = 0;
+    b = 1, W = 1, q = +1 gives E = N0 N1 + N0 + N1; with the weights negated
+    and maximize, E is that problem's own energy (the sign is restored)."""
+    N0, N1 = 7, 4
+    for opt, sgn in ((0, 1.0), (1, -1.0)):
+        an = oracle.BipartiteGraphAnnealer(np.float64)
+        an.seed(1)
+        an.set_qubo(sgn * np.ones(N0), sgn * np.ones(N1), sgn * np.ones((N1, N0)), optimize=opt)
+        an.set_m(1)
+        an.prepare()
+        an.set_q(np.ones((1, N0), np.int8), np.ones((1, N1), np.int8))
+        assert an.calculate_E().tolist() == [sgn * (N0 * N1 + N0 + N1)]
+        an.set_q(-np.ones((1, N0), np.int8), -np.ones((1, N1), np.int8))
+        assert an.calculate_E().tolist() == [0.0]
+
+
+def test_bipartite_randomize_spin_consumes_serial_stream(oracle):
+    """randomizeSpin with one worker (CPUBipartiteGraphAnnealer.cpp:172-192):
+    per slice, N0 draws for the q0 row, then N1 for the q1 row."""
+    N0, N1, m = 6, 5, 3
+    an = oracle.BipartiteGraphAnnealer(np.float32)
+    an.seed(42)
+    an.set_qubo(np.ones(N0, np.float32), np.ones(N1, np.float32), np.ones((N1, N0), np.float32))
+    an.set_m(m)
+    an.prepare()
+    an.randomize_spin()
+    q0, q1 = an.get_q()
+    w = np.where(oracle.mt_stream(42, m * (N0 + N1)) % 2 == 1, 1, -1).reshape(m, N0 + N1)
+    assert np.array_equal(q0, w[:, :N0]) and np.array_equal(q1, w[:, N0:])
+
+
+def _bg_action(h0, h1, J, c, q0, q1, coef):
+    """Suzuki-Trotter action of the bipartite model: mean_y E_y +
+    (coef/2) sum_y (q0_y . q0_(y+1) + q1_y . q1_(y+1))."""
+    E = -(q0 @ h0 + q1 @ h1 + np.einsum("yi,ij,yj->y", q1, J, q0)) - c
+    dot = np.sum(q0 * np.roll(q0, -1, axis=0)) + np.sum(q1 * np.roll(q1, -1, axis=0))
+    return E.mean() + 0.5 * coef * dot
+
+
+@pytest.mark.parametrize("m", [4, 5])
+def test_bipartite_sqa_zero_temperature_reaches_local_minimum_of_trotter_action(oracle, m):
+    """tryFlip of the bipartite half-steps (:331-344: dE = (2/m) q (h + dEmat)
+    - q (q_prev + q_next) coef, no factor 2 on dEmat) is the change of the
+    Trotter action; at beta = 1000 with coef = -3/16 every non-zero |dE| is at
+    least 1/10, so the final state is a local minimum over every single flip
+    of both parts.  Pins 2/m, the missing factor 2, J against J^T, the
+    neighbour slices (m odd: the ring closes between two even slices) and the
+    sign of the coupling."""
+    for seed in (3, 4, 5):
+        N0, N1, beta, coef = 9, 7, 1000.0, -3.0 / 16
+        G = np.arctanh(np.exp(coef * beta)) * m / beta
+        b0, b1, W = _bg_int_problem(N0, N1, 50 + seed)
+        an = oracle.BipartiteGraphAnnealer(np.float64)
+        an.seed(seed)
+        an.set_qubo(b0, b1, W)
+        an.set_m(m)
+        an.prepare()
+        an.randomize_spin()
+        for _ in range(60):
+            an.anneal_one_step(G, beta)
+        h0, h1, J, c = an.get_hamiltonian()
+        q0, q1 = (v.astype(np.float64) for v in an.get_q())
+        S0 = _bg_action(h0, h1, J, c, q0, q1, coef)
+        for part in (0, 1):
+            q = (q0, q1)[part]
+            for y in range(m):
+                for i in range(q.shape[1]):
+                    qf = q.copy()
+                    qf[y, i] = -qf[y, i]
+                    args = (qf, q1) if part == 0 else (q0, qf)
+                    assert _bg_action(h0, h1, J, c, *args, coef) >= S0 - 1e-9
+
+
+def test_bipartite_sa_zero_temperature_reaches_local_minimum(oracle):
+    """SA coloring (:485-496, dE = 2 q (h + dEmat)) at kT -> 0 ends in a state
+    no single flip improves, slice by slice."""
+    for seed in (1, 2):
+        N0, N1, m = 10, 8, 3
+        b0, b1, W = _bg_int_problem(N0, N1, 70 + seed)
+        an = oracle.BipartiteGraphAnnealer(np.float64)
+        an.seed(seed)
+        an.set_qubo(b0, b1, W)
+        an.set_m(m)
+        an.set_algorithm(7)
+        an.prepare()
+        an.randomize_spin()
+        for _ in range(30):
+            an.anneal_one_step(1e-9, 0.0)
+        h0, h1, J, c = an.get_hamiltonian()
+        q0, q1 = (v.astype(np.float64) for v in an.get_q())
+        E = lambda a, b: -(a @ h0 + b @ h1 + b @ J @ a) - c
+        for y in range(m):
+            E0 = E(q0[y], q1[y])
+            for i in range(N0):
+                a = q0[y].copy()
+                a[i] = -a[i]
+                assert E(a, q1[y]) >= E0
+            for i in range(N1):
+                b = q1[y].copy()
+                b[i] = -b[i]
+                assert E(q0[y], b) >= E0
+
+
+def test_bipartite_trivial_problems_reach_zero(oracle):
+    """test_bipartite_graph_annealer.py:196-266: b = 1, W = 1 anneals to E = 0
+    (minimize; maximize with the signs flipped), coloring m = 6 and SA m = 1."""
+    N0, N1 = 10, 8
+    for algo, m, (Gi, Gf, beta) in ((3, 6, (5.0, 0.02, 1 / 0.03)), (7, 1, (10.0, 0.02, 1.0))):
+        for opt, sgn in ((0, 1.0), (1, -1.0)):
+            an = oracle.BipartiteGraphAnnealer(np.float32)
+            an.seed(12)
+            an.set_qubo(sgn * np.ones(N0), sgn * np.ones(N1), sgn * np.ones((N1, N0)), optimize=opt)
+            an.set_m(m)
+            an.set_algorithm(algo)
+            an.prepare()
+            an.randomize_spin()
+            G, tau = Gi, np.exp(np.log(Gf / Gi) / 100)
+            for _ in range(100):
+                an.anneal_one_step(G, beta)
+                G *= tau
+            E = an.calculate_E()
+            assert (E.min() if opt == 0 else E.max()) == 0.0
