@@ -28,6 +28,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dense_sweep_bf16x2_n4096_m256.md")
+
+
+def measured_traffic(N, m, form):
+    """HBM bytes per sweep launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB copy) of this exact
+    workload; None for any other workload or form."""
+    if (N, m, form) != (4096, 256, "bf16x2") or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("sweep_hbm_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
 BETA = 50.0
 
 
@@ -210,10 +223,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(N, m, head),
             "kernel": "denseSweepKernel",
-            "note": f"algorithmic bytes per launch = N*m*(4N+N+8) = {per_launch} (reference-algorithm "
-                    f"work); the local-field sweep moves far fewer bytes, see profiles/",
+            "note": f"achieved = algorithmic bytes per launch N*m*(4N+N+8) = {per_launch} (the reference "
+                    f"algorithm's J-row + q-row + stream per attempt) / measured sweep time; traffic = "
+                    f"measured HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, profiles/)",
         },
         "forms": results,
     }
