@@ -101,8 +101,6 @@ def run_form(rocm, W, m, form, steps, warmup, seed):
         "attempts_per_s": attempts / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
         "sweep_avg_ms": avg_sweep_ms,
-        "refresh_ms_total": phases["refresh"][0],
-        "rng_ms_total": phases["rng"][0],
         "acceptance": flips / attempts,
         "attempts_per_batch": attempts / max(batches, 1),
         "E_min": float(np.min(E)),

@@ -86,12 +86,14 @@ struct PinnedBuf {
     PinnedBuf(const PinnedBuf &) = delete;
     PinnedBuf &operator=(const PinnedBuf &) = delete;
     ~PinnedBuf() { release(); }
-    void allocate(size_t count) {
+    /* flags: hipHostMallocDefault, or hipHostMallocCoherent | hipHostMallocMapped
+     * for a flag the device writes and the host polls */
+    void allocate(size_t count, unsigned flags = hipHostMallocDefault) {
         if (count <= n && ptr)
             return;
         release();
         if (count) {
-            HIP_CHECK(hipHostMalloc((void **)&ptr, count * sizeof(T), hipHostMallocDefault));
+            HIP_CHECK(hipHostMalloc((void **)&ptr, count * sizeof(T), flags));
             n = count;
         }
     }

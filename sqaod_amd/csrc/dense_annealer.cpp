@@ -142,8 +142,6 @@ DenseGraphAnnealer<real>::~DenseGraphAnnealer() {
         (void)hipSetDevice(dev_->devno());
         (void)hipStreamSynchronize(dev_->stream());
     }
-    if (abortEvent_)
-        (void)hipEventDestroy(abortEvent_);
     for (auto &pe : pending_) {
         (void)hipEventDestroy(pe.start);
         (void)hipEventDestroy(pe.stop);
@@ -322,7 +320,11 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
     } else if (name == "refresh_interval") {
         refreshInterval_ = atoi(value.c_str());
     } else if (name == "profile") {
-        profile_ = (value == "1" || value == "on" || value == "true" || value == "True");
+        /* 0: off; 1: HIP events around each sweep; 2: around every phase */
+        if (value == "on" || value == "true" || value == "True")
+            profile_ = 1;
+        else
+            profile_ = atoi(value.c_str()) < 0 ? 0 : atoi(value.c_str());
     } else if (name == "spin_limit") {
         spinLimit_ = atoi(value.c_str());
     } else if (name == "precision" || name == "device") {
@@ -348,7 +350,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("slices_per_block", std::to_string(slicesPerBlock_));
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
     p.emplace_back("refresh_engine", refreshVector_ ? "vector" : "mfma");
-    p.emplace_back("profile", profile_ ? "1" : "0");
+    p.emplace_back("profile", std::to_string(profile_));
     return p;
 }
 
@@ -472,10 +474,9 @@ void DenseGraphAnnealer<real>::prepare() {
     d_counters_.allocate(16); /* [0..2] public, [3..8] optional per-phase cycles */
     HIP_CHECK(hipMemsetAsync(d_abort_.ptr, 0, sizeof(int), dev_->stream()));
     HIP_CHECK(hipMemsetAsync(d_counters_.ptr, 0, 16 * sizeof(unsigned long long), dev_->stream()));
-    h_abort_.allocate(1);
+    h_abort_.allocate(1, hipHostMallocCoherent | hipHostMallocMapped);
     h_abort_.ptr[0] = 0;
-    if (!abortEvent_)
-        HIP_CHECK(hipEventCreateWithFlags(&abortEvent_, hipEventDisableTiming));
+    HIP_CHECK(hipHostGetDevicePointer((void **)&d_abortHost_, h_abort_.ptr, 0));
     abortPending_ = false;
     if (rngMT_) {
         d_mt_.allocate(1);
@@ -542,14 +543,21 @@ void DenseGraphAnnealer<real>::set_qset(const int8_t *q, int m, int N) {
 
 template <class real>
 void DenseGraphAnnealer<real>::checkAbort(bool synchronous) {
+    /* The sweep writes the abort flag into pinned host memory.  Without a
+     * synchronisation (an annealOneStep call) a flag already written is
+     * reported early; synchronous callers have synchronised the stream first,
+     * so every finished sweep is seen. */
     if (!abortPending_)
         return;
+    const int flag = __atomic_load_n(h_abort_.ptr, __ATOMIC_ACQUIRE);
     if (synchronous)
-        HIP_CHECK(hipEventSynchronize(abortEvent_));
-    else if (hipEventQuery(abortEvent_) != hipSuccess)
-        return;
-    abortPending_ = false;
-    if (h_abort_.ptr[0]) {
+        abortPending_ = false;
+    if (flag) {
+        dev_->synchronize();
+        abortPending_ = false;
+        h_abort_.ptr[0] = 0;
+        HIP_CHECK(hipMemsetAsync(d_abort_.ptr, 0, sizeof(int), dev_->stream()));
+        dev_->synchronize();
         clearState(solQSet);
         SQ_THROW("sweep aborted: a slice waited past the spin limit on a neighbour's decision; spins are invalid");
     }
@@ -618,16 +626,16 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
         fieldsDirty_ = true;
     PhaseEvents pe;
     if (fieldsDirty_) {
-        if (profile_)
+        if (profile_ >= 2)
             phaseBegin(0, pe);
         refreshFields();
-        if (profile_)
+        if (profile_ >= 2)
             phaseEnd(pe);
     }
-    if (profile_)
+    if (profile_ >= 2)
         phaseBegin(1, pe);
     generateAttempts();
-    if (profile_)
+    if (profile_ >= 2)
         phaseEnd(pe);
 
     SweepArgs<real> a{};
@@ -655,6 +663,7 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
         HIP_CHECK(hipMemsetAsync(d_D_.ptr, 0, d_D_.bytes(), dev_->stream()));
     }
     a.abortFlag = d_abort_.ptr;
+    a.abortHost = d_abortHost_;
     a.counters = d_counters_.ptr;
     a.spinLimit = spinLimit_;
 
@@ -675,9 +684,7 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     if (pending_.size() > 3000)
         collectPhases();
 
-    HIP_CHECK(hipMemcpyAsync(h_abort_.ptr, d_abort_.ptr, sizeof(int), hipMemcpyDeviceToHost, dev_->stream()));
-    HIP_CHECK(hipEventRecord(abortEvent_, dev_->stream()));
-    abortPending_ = true;
+    abortPending_ = true; /* the sweep reports an abort in h_abort_ itself */
     ++stepsSinceRefresh_;
     clearState(solSolutionAvailable);
 }

@@ -81,6 +81,14 @@ __host__ __device__ inline int pendingTableSize(int Np) {
     return t;
 }
 
+/* Pending-site table codes: the serially first undecided neighbour attempt
+ * registered at a slot, as ((e - Kk + N) << 1) | nb (unique per attempt),
+ * kPendMulti when more than one attempt hashed there, kPendEmpty when none. */
+constexpr int kPendEmpty = 0x7fffffff;
+constexpr int kPendMulti = 1 << 30;
+__device__ __forceinline__ int pendCode(int rel, int N, int nb) { return ((rel + N) << 1) | nb; }
+__device__ __forceinline__ int pendRel(int code, int N) { return ((code & (kPendMulti - 1)) >> 1) - N; }
+
 __device__ __forceinline__ int slicePos(int y, int m) {
     int nEven = (m + 1) / 2;
     return (y & 1) ? nEven + (y >> 1) : (y >> 1);
@@ -183,7 +191,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
         for (int i = lane; i < Np; i += kWave)
             F[i] = a.F[(long long)y * Np + i];
     for (int i = lane; i < tblN; i += kWave)
-        tbl[i] = 0x7fffffff;
+        tbl[i] = kPendEmpty;
     for (int i = lane; i < Nw; i += kWave) {
         Qb[i] = a.Qbits[(long long)y * Nw + i];
         Mir[0][i] = a.Qbits[(long long)n0 * Nw + i];
@@ -296,13 +304,14 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
                 const int A = applied[nb];
                 const int base = nbase[nb];
                 const int e0 = base + lane, e1 = base + kWave + lane;
-                /* an entry blocks lane i only if e - Kk < i <= 63 */
+                /* an entry matters to lane i only if e - Kk < i <= 63; the slot
+                 * keeps the serially first entry, encoded uniquely per (e, nb) */
                 pendSet[nb][0] = e0 >= A && e0 < N && e0 - Kk < kWave - 1;
                 pendSet[nb][1] = e1 >= A && e1 < N && e1 - Kk < kWave - 1;
                 if (pendSet[nb][0])
-                    atomicMin(&tbl[ns[nb][0] & tblMask], e0 - Kk);
+                    atomicMin(&tbl[ns[nb][0] & tblMask], pendCode(e0 - Kk, N, nb));
                 if (pendSet[nb][1])
-                    atomicMin(&tbl[ns[nb][1] & tblMask], e1 - Kk);
+                    atomicMin(&tbl[ns[nb][1] & tblMask], pendCode(e1 - Kk, N, nb));
                 /* lane i needs entries [A, Kk + i): they must lie in the window */
                 const int winEnd = base + 2 * kWave < N ? base + 2 * kWave : N;
                 if (winEnd < N) {
@@ -312,34 +321,67 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             }
         }
         waveSync();
+        /* a slot written by more than one entry is marked: lanes meeting it
+         * wait instead of deciding under both outcomes of one entry */
+        int curBase[2] = {nbase[0], nbase[1]};
+        if (!a.sa) {
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const int Kk = k + delta[nb];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int e = curBase[nb] + h * kWave + lane;
+                    if (pendSet[nb][h]) {
+                        int *slot = &tbl[ns[nb][h] & tblMask];
+                        if (*slot != pendCode(e - Kk, N, nb))
+                            atomicOr(slot, kPendMulti);
+                    }
+                }
+            }
+            waveSync();
+        }
+        /* the next batch's neighbour windows start where this batch's replay
+         * ended: fetch them now, so the loads overlap evaluation and commit */
+        bool pendSetNow[2][2] = {{pendSet[0][0], pendSet[0][1]}, {pendSet[1][0], pendSet[1][1]}};
+        uint32_t pendSite[2][2] = {{ns[0][0], ns[0][1]}, {ns[1][0], ns[1][1]}};
+        if (!a.sa) {
+            loadNeighbour(0);
+            loadNeighbour(1);
+        }
         SQ_CYC(c2);
         SQ_CYC_ADD(cyc[1], c1, c2);
 
         /* evaluate attempt k + lane speculatively */
         const bool valid = lane < limit;
         bool blocked = false;
-        if (!a.sa)
-            blocked = tbl[s & tblMask] < lane;
+        int hypNb = -1; /* one undecided neighbour attempt precedes this one at this slot */
+        if (!a.sa) {
+            const int code = tbl[s & tblMask];
+            if (code != kPendEmpty && pendRel(code, N) < lane) {
+                if (code & kPendMulti)
+                    blocked = true;
+                else
+                    hypNb = code & 1;
+            }
+        }
         const uint32_t qword = Qb[s >> 5];
         const bool qpos = (qword >> (s & 31)) & 1u;
         const real q = qpos ? (real)1 : (real)-1;
         const real hx = hS[s];
-        real nsum = 0;
+        real nsum = 0, nsumAlt = 0;
         if (!a.sa) {
             const real nv0 = ((Mir[0][s >> 5] >> (s & 31)) & 1u) ? (real)1 : (real)-1;
             const real nv1 = ((Mir[1][s >> 5] >> (s & 31)) & 1u) ? (real)1 : (real)-1;
             nsum = nv0 + nv1;
+            /* the sum if the pending neighbour attempt flips this site */
+            nsumAlt = hypNb == 0 ? -nv0 + nv1 : nv0 - nv1;
         }
-        /* the pending entries of this batch are now in the table; the next
-         * batch's neighbour windows can be fetched while this one evaluates */
-        bool pendSetNow[2][2] = {{pendSet[0][0], pendSet[0][1]}, {pendSet[1][0], pendSet[1][1]}};
-        uint32_t pendSite[2][2] = {{ns[0][0], ns[0][1]}, {ns[1][0], ns[1][1]}};
-        auto decide = [&](real f) {
+        auto decideWith = [&](real f, real ns_) {
             real thr;
             if (!a.sa) {
                 /* CPUDenseGraphAnnealer.cpp:258-262, same operation order */
                 real dE = a.twoDivM * q * (hx + (real)2. * f);
-                dE -= q * nsum * a.coef;
+                dE -= q * ns_ * a.coef;
                 thr = (dE < (real)0.) ? (real)1. : Expo<real>::eval(-dE * a.beta, expTab);
             } else {
                 /* :348-349 */
@@ -348,6 +390,18 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             }
             return thr > u;
         };
+        auto decide = [&](real f) { return decideWith(f, nsum); };
+        bool accLocal = false;
+        if constexpr (PLAIN) {
+            blocked = blocked || hypNb >= 0;
+        } else {
+            /* decided under both outcomes of the pending attempt: if they
+             * agree the decision is final, otherwise the lane waits */
+            const real f = F[s];
+            accLocal = decide(f);
+            if (hypNb >= 0 && decideWith(f, nsumAlt) != accLocal)
+                blocked = true;
+        }
         const unsigned long long stopMask = __ballot(!valid || blocked);
         const int firstStop = firstLane(stopMask);
         int c = firstStop, flipLane = -1;
@@ -394,7 +448,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             if (flipLane >= 0)
                 c = flipLane + 1;
         } else {
-            const bool acc = decide(F[s]);
+            const bool acc = accLocal;
             const unsigned long long accMask = __ballot(valid && !blocked && acc);
             const int firstAcc = firstLane(accMask);
             if (firstAcc < firstStop) {
@@ -411,9 +465,9 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
                 if (pendSetNow[nb][0])
-                    tbl[pendSite[nb][0] & tblMask] = 0x7fffffff;
+                    tbl[pendSite[nb][0] & tblMask] = kPendEmpty;
                 if (pendSetNow[nb][1])
-                    tbl[pendSite[nb][1] & tblMask] = 0x7fffffff;
+                    tbl[pendSite[nb][1] & tblMask] = kPendEmpty;
             }
         }
 
@@ -427,7 +481,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
                 int waitNb = -1, waitE = 0x7fffffff;
 #pragma unroll
                 for (int nb = 0; nb < 2; ++nb) {
-                    const int base = nbase[nb];
+                    const int base = curBase[nb];
                     const int Kk = k + delta[nb];
                     const int e0 = base + lane, e1 = base + kWave + lane;
                     const bool m0 = pendSetNow[nb][0] && pendSite[nb][0] == s0 && e0 < Kk;
@@ -452,6 +506,8 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             }
             if (++spins > a.spinLimit) {
                 atomicExch(a.abortFlag, 1);
+                if (a.abortHost) /* a vector store, system scope: the host reads it after the step */
+                    __hip_atomic_store(a.abortHost, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 aborted = true;
                 break;
             }
@@ -514,10 +570,6 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             }
         }
         k += c;
-        if (!a.sa && k < N) {
-            loadNeighbour(0);
-            loadNeighbour(1);
-        }
         waveSync();
         SQ_CYC(c5);
         SQ_CYC_ADD(cyc[3], c3, c5);

@@ -169,6 +169,7 @@ struct SweepArgs {
     uint32_t *D;            /* m x N decisions (tag << 1 | flipped) */
     uint32_t tag;
     int *abortFlag;         /* set by a wave whose wait exceeds the spin limit */
+    int *abortHost;         /* host-visible copy of abortFlag (pinned host memory), or null */
     unsigned long long *counters; /* [0] accepted flips, [1] batches, [2] blocked waits */
     int spinLimit;
 };

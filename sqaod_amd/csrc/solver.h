@@ -171,7 +171,7 @@ private:
     bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
     int spinLimit_ = 1 << 20;
 
-    bool profile_ = false;        /* "profile": time phases with HIP events */
+    int profile_ = 0;             /* "profile": 1 = HIP events around each sweep, 2 = around every phase */
     struct PhaseEvents {
         int phase;
         hipEvent_t start, stop;
@@ -202,7 +202,7 @@ private:
     DevBuf<unsigned long long> d_counters_;
     DevBuf<int8_t> d_q8_;
     PinnedBuf<int> h_abort_;
-    hipEvent_t abortEvent_ = nullptr;
+    int *d_abortHost_ = nullptr; /* device address of h_abort_ */
     bool abortPending_ = false;
 
     /* host copies */
