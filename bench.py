@@ -2,11 +2,16 @@
 """Dense SQA throughput on one MI355X: spin-flip attempts/s of annealOneStep at
 N=4096 spins, m=256 Trotter slices (W symmetric i.i.d. N(0,1), fp32), over the
 annealing schedule G = 5.0 * 0.97^k, beta = 50, one sweep (N*m attempts) per
-step.  Three forms of the rocm dense annealer are timed on the same instance:
+step.  Four forms of the rocm dense annealer are timed on the same instance:
 
   bf16x2  local fields in LDS, J rows / field GEMM as bf16 hi+lo, fp32 fields (headline)
   fp32    local fields in LDS, fp32 J
+  fp8     local fields in LDS, J as e4m3 with a power-of-two scale per row
+          (1-byte J rows, fp8 MFMA field GEMM), fp32 fields
   plain   the reference algorithm: a J-row . q-row dot per attempt, fp32
+
+Every form reports the FP64 energies of its final spins and the max
+|E_form - E| of the J it anneals against the given J (both FP64).
 
 Each form runs W warm-up steps, re-randomizes the spins, then K timed steps
 bracketed by device synchronisation.  The sweep kernel's average duration is
@@ -60,6 +65,8 @@ def make_W(N, seed):
 def bytes_per_attempt(N, form):
     """Reference-algorithm bytes of one attempt: one J row, one q row (int8),
     8 bytes of random stream (site + uniform)."""
+    if form == "fp8":
+        return N * 1 + 4 + N * 1 + 8  # e4m3 row + its 4 B scale
     bJ = 4  # fp32 and bf16 hi+lo both 4 B per element
     return N * bJ + N * 1 + 8
 
@@ -95,6 +102,7 @@ def run_form(rocm, W, m, form, steps, warmup, seed):
     avg_sweep_ms = sweep_ms / max(sweep_n, 1)
     E = an.get_E()
     Eerr = an.get_energy_rounding_error()
+    Eform = an.get_form_energy_error()
     flips = c1["flips"] - c0["flips"]
     batches = c1["batches"] - c0["batches"]
     res = {
@@ -106,9 +114,112 @@ def run_form(rocm, W, m, form, steps, warmup, seed):
         "E_min": float(np.min(E)),
         "E_mean": float(np.mean(E)),
         "E_fp64_rounding_max_abs": Eerr,
+        "E_form_max_abs_err": Eform,
+        "E_form_max_rel_err": Eform / max(abs(float(np.min(E))), 1e-30),
     }
     del an
     return res
+
+
+def run_bipartite(rocm, steps, form, seed=1133557):
+    """Bipartite SQA at N0 = N1 = 1024, m = 64, b0, b1, W ~ N(0,1) (fp32), the
+    same schedule; m (N0 + N1) = 131,072 attempts per step."""
+    N0 = N1 = 1024
+    m = 64
+    rng = np.random.default_rng(seed)
+    b0 = rng.standard_normal(N0).astype(np.float32)
+    b1 = rng.standard_normal(N1).astype(np.float32)
+    W = rng.standard_normal((N1, N0)).astype(np.float32)
+    an = rocm.bipartite_graph_annealer(b0, b1, W, dtype=np.float32, n_trotters=m, field_precision=form)
+    dev = rocm.device.get_active_device()
+    an.seed(1000)
+    an.prepare()
+    an.randomize_spin()
+    for k in range(5):
+        an.anneal_one_step(schedule_G(k), BETA)
+    an.randomize_spin()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        an.anneal_one_step(schedule_G(k), BETA)
+    dev.synchronize()
+    el = time.perf_counter() - t0
+    E = an.get_E()
+    return {"attempts_per_s": steps * m * (N0 + N1) / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
+            "E_min": float(np.min(E))}
+
+
+def run_bf(rocm, N, seed=1133557):
+    """Dense brute-force search over all 2^N x, W symmetric integer in [-8, 8]."""
+    rng = np.random.default_rng(seed + N)
+    U = rng.integers(-8, 9, size=(N, N)).astype(np.float32)
+    W = np.triu(U) + np.triu(U, 1).T
+    s = rocm.dense_graph_bf_searcher(W, dtype=np.float32)
+    s.search()  # warm-up (allocations, code objects)
+    t0 = time.perf_counter()
+    s.search()
+    el = time.perf_counter() - t0
+    return {"N": N, "candidates_per_s": float(2 ** N) / el, "ms": 1e3 * el, "E_min": float(s.get_E()[0]),
+            "n_solutions": len(s.get_x())}
+
+
+def run_serve(rocm, n_jobs, load, seed=20240601):
+    """Serving: jobs of 70% dense / 30% bipartite (N0 = N1 = N/2) problems,
+    N log-uniform in [32, 8192], m uniform in {16..256}, 50..500 steps of the
+    schedule, weights N(0,1).  Each job runs end to end on the device's stream
+    (set_qubo, prepare, randomize_spin, the steps, get_E) and its service time
+    is measured.  Arrivals are Poisson at the rate giving `load` utilisation;
+    latency = completion - arrival of a FIFO single-stream server replayed
+    over the measured service times."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n_jobs):
+        N = int(round(np.exp(rng.uniform(np.log(32), np.log(8192)))))
+        m = int(rng.integers(16, 257))
+        steps = int(rng.integers(50, 501))
+        if rng.random() < 0.7:
+            U = rng.standard_normal((N, N)).astype(np.float32)
+            W = np.triu(U) + np.triu(U, 1).T
+            jobs.append(("dense", N, m, steps, (W,)))
+        else:
+            h = max(N // 2, 1)
+            jobs.append(("bipartite", N, m, steps, (rng.standard_normal(h).astype(np.float32),
+                                                    rng.standard_normal(h).astype(np.float32),
+                                                    rng.standard_normal((h, h)).astype(np.float32))))
+    dev = rocm.device.get_active_device()
+    service, attempts = [], 0
+    for kind, N, m, steps, data in jobs:
+        t0 = time.perf_counter()
+        if kind == "dense":
+            an = rocm.dense_graph_annealer(data[0], dtype=np.float32, n_trotters=m)
+            per = N * m
+        else:
+            an = rocm.bipartite_graph_annealer(*data, dtype=np.float32, n_trotters=m)
+            per = m * 2 * (N // 2)
+        an.seed(7)
+        an.prepare()
+        an.randomize_spin()
+        for k in range(steps):
+            an.anneal_one_step(schedule_G(k), BETA)
+        an.get_E()
+        dev.synchronize()
+        service.append(time.perf_counter() - t0)
+        attempts += per * steps
+        del an
+    service = np.array(service)
+    rate = load / service.mean()  # arrivals per second
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    done, lat = 0.0, []
+    for a, sv in zip(arrivals, service):
+        start = max(a, done)
+        done = start + sv
+        lat.append(done - a)
+    lat = np.array(lat) * 1e3
+    return {"jobs": n_jobs, "qps": rate, "load": load, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "service_ms_mean": 1e3 * float(service.mean()),
+            "attempts_per_s": attempts / float(service.sum()),
+            "note": "FIFO single-stream server; latencies replayed over measured service times at Poisson "
+                    "arrivals of the stated rate"}
 
 
 def cpu_baseline(W, m, seconds, seed):
@@ -167,10 +278,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--N", type=int, default=4096)
     ap.add_argument("--m", type=int, default=256)
-    ap.add_argument("--forms", default="bf16x2,fp32,plain")
+    ap.add_argument("--forms", default="bf16x2,fp32,fp8,plain")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the bipartite, BF search and serving runs")
+    ap.add_argument("--serve-jobs", type=int, default=48)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -233,6 +346,16 @@ def main():
         out["speedup_vs_plain_fp32"] = r["attempts_per_s"] / results["plain"]["attempts_per_s"]
     if "fp32" in results and head in results:
         out["speedup_vs_local_fp32"] = r["attempts_per_s"] / results["fp32"]["attempts_per_s"]
+    if not args.no_extras:
+        out["bipartite"] = {f: run_bipartite(rocm, args.steps, f) for f in ("fp32", "bf16x2")}
+        print(f"# bipartite: {out['bipartite']['fp32']['attempts_per_s']:.4g} attempts/s", file=sys.stderr,
+              flush=True)
+        out["bf_search"] = [run_bf(rocm, n) for n in (24, 30)]
+        print(f"# bf search N=30: {out['bf_search'][1]['candidates_per_s']:.4g} candidates/s", file=sys.stderr,
+              flush=True)
+        out["serve"] = run_serve(rocm, args.serve_jobs, 0.7)
+        print(f"# serve: p50 {out['serve']['p50_ms']:.1f} ms, p99 {out['serve']['p99_ms']:.1f} ms", file=sys.stderr,
+              flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(W, m, args.cpu_seconds, seed=1000)
     print(json.dumps(out), flush=True)

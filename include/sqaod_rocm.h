@@ -34,6 +34,7 @@ typedef struct sqaod_rocm_device sqaod_rocm_device;
 typedef struct sqaod_rocm_dg_annealer sqaod_rocm_dg_annealer;
 typedef struct sqaod_rocm_dg_bf_searcher sqaod_rocm_dg_bf_searcher;
 typedef struct sqaod_rocm_bg_annealer sqaod_rocm_bg_annealer;
+typedef struct sqaod_rocm_formulas sqaod_rocm_formulas;
 
 /* ---- library ---- */
 
@@ -109,6 +110,11 @@ int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, 
  * max |E_dtype - E_fp64| of the last energy evaluation. */
 int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]);
 int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err);
+/* Diagnostic without a reference counterpart: max over slices of
+ * |E_form(q_y) - E(q_y)|, both in FP64, for the current spins, where E_form
+ * uses the J the field precision form anneals (bf16 hi + lo or the fp8 J')
+ * and E the J as given; 0 for the fp32/fp64 form. */
+int sqaod_rocm_dg_annealer_get_form_energy_error(sqaod_rocm_dg_annealer *a, double *err);
 /* Diagnostic without a reference counterpart: the local fields F[y][i] =
  * sum_j J[i][j] q[y][j] (m x N, dtype) the local-field sweep holds, to check
  * the field GEMM and its precision forms. */
@@ -202,11 +208,53 @@ int sqaod_rocm_bg_annealer_make_solution(sqaod_rocm_bg_annealer *a);
 int sqaod_rocm_bg_annealer_get_system_E(sqaod_rocm_bg_annealer *a, double G, double beta, double *E);
 int sqaod_rocm_bg_annealer_anneal_one_step(sqaod_rocm_bg_annealer *a, double G, double beta);
 
+/* ---- formulas: formulas.inc method table (:586-606).  One object serves the
+ * dense and the bipartite functions (dg_formulas_new and bg_formulas_new both
+ * map to sqaod_rocm_formulas_new); dtype per call as in the reference.  The
+ * single-vector functions are the batch functions with nBatch = 1.  Energies
+ * are formed on the device in FP64 and rounded to dtype once; Hamiltonians on
+ * the host in double. ---- */
+/* dg_formulas_new / bg_formulas_new, _delete, _assign_device (:27-73, :280-330) */
+int sqaod_rocm_formulas_new(sqaod_rocm_formulas **out);
+int sqaod_rocm_formulas_delete(sqaod_rocm_formulas *f);
+int sqaod_rocm_formulas_assign_device(sqaod_rocm_formulas *f, sqaod_rocm_device *dev);
+/* dense_graph_(batch_)calculate_E (:89-157): E[b] = x_b^T W x_b, x int8 nBatch x N */
+int sqaod_rocm_dense_graph_batch_calculate_E(sqaod_rocm_formulas *f, int dtype, void *E, const void *W, int N,
+                                             const int8_t *x, int nBatch);
+/* dense_graph_calculate_hamiltonian (:159-201): h[N], J[N x N], c[1] */
+int sqaod_rocm_dense_graph_calculate_hamiltonian(sqaod_rocm_formulas *f, int dtype, void *h, void *J, void *c,
+                                                 const void *W, int N);
+/* dense_graph_(batch_)calculate_E_from_spin (:203-276): E[b] = -c - h.q_b - q_b^T J q_b */
+int sqaod_rocm_dense_graph_batch_calculate_E_from_spin(sqaod_rocm_formulas *f, int dtype, void *E, const void *h,
+                                                       const void *J, const void *c, const int8_t *q, int N,
+                                                       int nBatch);
+/* bipartite_graph_(batch_)calculate_E (:346-420): E[b] = b0.x0_b + b1.x1_b + x1_b^T W x0_b, W N1 x N0 */
+int sqaod_rocm_bipartite_graph_batch_calculate_E(sqaod_rocm_formulas *f, int dtype, void *E, const void *b0, int N0,
+                                                 const void *b1, int N1, const void *W, const int8_t *x0,
+                                                 const int8_t *x1, int nBatch);
+/* bipartite_graph_batch_calculate_E_2d (:422-462): E[nBatch1 x nBatch0] over all (x0, x1) pairs */
+int sqaod_rocm_bipartite_graph_batch_calculate_E_2d(sqaod_rocm_formulas *f, int dtype, void *E, const void *b0,
+                                                    int N0, const void *b1, int N1, const void *W, const int8_t *x0,
+                                                    int nBatch0, const int8_t *x1, int nBatch1);
+/* bipartite_graph_calculate_hamiltonian (:464-510): h0[N0], h1[N1], J[N1 x N0], c[1] */
+int sqaod_rocm_bipartite_graph_calculate_hamiltonian(sqaod_rocm_formulas *f, int dtype, void *h0, void *h1, void *J,
+                                                     void *c, const void *b0, int N0, const void *b1, int N1,
+                                                     const void *W);
+/* bipartite_graph_(batch_)calculate_E_from_spin (:512-578): E[b] = -(h0.q0 + h1.q1 + q1^T J q0) - c */
+int sqaod_rocm_bipartite_graph_batch_calculate_E_from_spin(sqaod_rocm_formulas *f, int dtype, void *E, const void *h0,
+                                                           int N0, const void *h1, int N1, const void *J,
+                                                           const void *c, const int8_t *q0, const int8_t *q1,
+                                                           int nBatch);
+
 /* Self check without a reference counterpart: order-independent checksum of
  * the device exp (the acceptance test's expf) over float bit patterns
  * [lo, hi), for comparison with the host C library's expf:
  * sum of splitmix64((b << 32) | bits(expf(b))), NaN canonicalised. */
 int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsigned long long *out);
+
+/* Self check without a reference counterpart: the fp8 sweep's e4m3 decode
+ * (gfx950 v_cvt_pk_f32_fp8) applied to the byte codes 0..255 (out[256]). */
+int sqaod_rocm_fp8_decode_table(float *out);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,24 @@ void withBf(sqaod_rocm_dg_bf_searcher *h, Fn &&fn) {
         fn(*bh->d);
 }
 
+struct FormulasHandle {
+    Formulas<float> f;
+    Formulas<double> d;
+};
+
+template <class Fn>
+void withFormulas(sqaod_rocm_formulas *h, int dtype, Fn &&fn) {
+    SQ_THROW_IF(h == nullptr, "null formulas handle");
+    SQ_THROW_IF(dtype != 0 && dtype != 1, "dtype must be 0 (float32) or 1 (float64)");
+    FormulasHandle *fh = reinterpret_cast<FormulasHandle *>(h);
+    if (dtype == 0)
+        fn(fh->f);
+    else
+        fn(fh->d);
+}
+
+#define SQ_REAL(s) typename std::remove_reference_t<decltype(s)>::real_type
+
 struct BgHandle {
     int dtype;
     BipartiteGraphAnnealer<float> *f = nullptr;
@@ -382,11 +400,23 @@ int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, 
     return guarded([&] { withDg(a, [&](auto &s) { *err = s.lastEnergyRoundingError(); }); });
 }
 
+int sqaod_rocm_dg_annealer_get_form_energy_error(sqaod_rocm_dg_annealer *a, double *err) {
+    return guarded([&] { withDg(a, [&](auto &s) { *err = s.formEnergyError(); }); });
+}
+
 int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsigned long long *out) {
     return guarded([&] {
         SQ_THROW_IF(!out || lo > hi || hi > 0x100000000ull, "bad range");
         defaultDevice().makeCurrent();
         *out = expfChecksum(lo, hi);
+    });
+}
+
+int sqaod_rocm_fp8_decode_table(float *out) {
+    return guarded([&] {
+        SQ_THROW_IF(!out, "null output");
+        defaultDevice().makeCurrent();
+        fp8DecodeTable(out);
     });
 }
 
@@ -687,6 +717,119 @@ int sqaod_rocm_bg_annealer_anneal_one_step(sqaod_rocm_bg_annealer *a, double G, 
         withBg(a, [&](auto &s) {
             using real = typename std::remove_reference_t<decltype(s)>::real_type;
             s.annealOneStep((real)G, (real)beta);
+        });
+    });
+}
+
+/* ---------------- formulas ---------------- */
+
+int sqaod_rocm_formulas_new(sqaod_rocm_formulas **out) {
+    return guarded([&] {
+        SQ_THROW_IF(out == nullptr, "null output pointer");
+        *out = reinterpret_cast<sqaod_rocm_formulas *>(new FormulasHandle);
+    });
+}
+
+int sqaod_rocm_formulas_delete(sqaod_rocm_formulas *f) {
+    return guarded([&] { delete reinterpret_cast<FormulasHandle *>(f); });
+}
+
+int sqaod_rocm_formulas_assign_device(sqaod_rocm_formulas *f, sqaod_rocm_device *dev) {
+    return guarded([&] {
+        SQ_THROW_IF(!f || !dev, "null handle");
+        FormulasHandle *fh = reinterpret_cast<FormulasHandle *>(f);
+        fh->f.assignDevice(*dev->dev);
+        fh->d.assignDevice(*dev->dev);
+    });
+}
+
+int sqaod_rocm_dense_graph_batch_calculate_E(sqaod_rocm_formulas *f, int dtype, void *E, const void *W, int N,
+                                             const int8_t *x, int nBatch) {
+    return guarded([&] {
+        SQ_THROW_IF(!E || !W || !x, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.denseCalculateE(static_cast<real *>(E), static_cast<const real *>(W), N, x, nBatch);
+        });
+    });
+}
+
+int sqaod_rocm_dense_graph_calculate_hamiltonian(sqaod_rocm_formulas *f, int dtype, void *h, void *J, void *c,
+                                                 const void *W, int N) {
+    return guarded([&] {
+        SQ_THROW_IF(!h || !J || !c || !W, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.denseHamiltonian(static_cast<real *>(h), static_cast<real *>(J), static_cast<real *>(c),
+                               static_cast<const real *>(W), N);
+        });
+    });
+}
+
+int sqaod_rocm_dense_graph_batch_calculate_E_from_spin(sqaod_rocm_formulas *f, int dtype, void *E, const void *h,
+                                                       const void *J, const void *c, const int8_t *q, int N,
+                                                       int nBatch) {
+    return guarded([&] {
+        SQ_THROW_IF(!E || !h || !J || !c || !q, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.denseCalculateEFromSpin(static_cast<real *>(E), static_cast<const real *>(h),
+                                      static_cast<const real *>(J), *static_cast<const real *>(c), q, N, nBatch);
+        });
+    });
+}
+
+int sqaod_rocm_bipartite_graph_batch_calculate_E(sqaod_rocm_formulas *f, int dtype, void *E, const void *b0, int N0,
+                                                 const void *b1, int N1, const void *W, const int8_t *x0,
+                                                 const int8_t *x1, int nBatch) {
+    return guarded([&] {
+        SQ_THROW_IF(!E || !b0 || !b1 || !W || !x0 || !x1, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.bipartiteCalculateE(static_cast<real *>(E), static_cast<const real *>(b0), static_cast<const real *>(b1),
+                                  static_cast<const real *>(W), N0, N1, x0, nBatch, x1, nBatch, false);
+        });
+    });
+}
+
+int sqaod_rocm_bipartite_graph_batch_calculate_E_2d(sqaod_rocm_formulas *f, int dtype, void *E, const void *b0,
+                                                    int N0, const void *b1, int N1, const void *W, const int8_t *x0,
+                                                    int nBatch0, const int8_t *x1, int nBatch1) {
+    return guarded([&] {
+        SQ_THROW_IF(!E || !b0 || !b1 || !W || !x0 || !x1, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.bipartiteCalculateE(static_cast<real *>(E), static_cast<const real *>(b0), static_cast<const real *>(b1),
+                                  static_cast<const real *>(W), N0, N1, x0, nBatch0, x1, nBatch1, true);
+        });
+    });
+}
+
+int sqaod_rocm_bipartite_graph_calculate_hamiltonian(sqaod_rocm_formulas *f, int dtype, void *h0, void *h1, void *J,
+                                                     void *c, const void *b0, int N0, const void *b1, int N1,
+                                                     const void *W) {
+    return guarded([&] {
+        SQ_THROW_IF(!h0 || !h1 || !J || !c || !b0 || !b1 || !W, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.bipartiteHamiltonian(static_cast<real *>(h0), static_cast<real *>(h1), static_cast<real *>(J),
+                                   static_cast<real *>(c), static_cast<const real *>(b0), static_cast<const real *>(b1),
+                                   static_cast<const real *>(W), N0, N1);
+        });
+    });
+}
+
+int sqaod_rocm_bipartite_graph_batch_calculate_E_from_spin(sqaod_rocm_formulas *f, int dtype, void *E, const void *h0,
+                                                           int N0, const void *h1, int N1, const void *J,
+                                                           const void *c, const int8_t *q0, const int8_t *q1,
+                                                           int nBatch) {
+    return guarded([&] {
+        SQ_THROW_IF(!E || !h0 || !h1 || !J || !c || !q0 || !q1, "null input");
+        withFormulas(f, dtype, [&](auto &s) {
+            using real = SQ_REAL(s);
+            s.bipartiteCalculateEFromSpin(static_cast<real *>(E), static_cast<const real *>(h0),
+                                          static_cast<const real *>(h1), static_cast<const real *>(J),
+                                          *static_cast<const real *>(c), N0, N1, q0, q1, nBatch);
         });
     });
 }

@@ -294,17 +294,19 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
             f = fieldF32;
         else if (value == "bf16x2")
             f = fieldBF16x2;
+        else if (value == "fp8")
+            f = fieldFP8;
         else
-            SQ_THROW("unknown field_precision '%s' (fp32 | bf16x2)", value.c_str());
-        SQ_THROW_IF(f == fieldBF16x2 && sizeof(real) != 4, "bf16x2 fields are available for float32 solvers");
+            SQ_THROW("unknown field_precision '%s' (fp32 | bf16x2 | fp8)", value.c_str());
+        SQ_THROW_IF(f != fieldF32 && sizeof(real) != 4, "%s fields are available for float32 solvers",
+                    value.c_str());
         if (f != fieldFormat_) {
             fieldFormat_ = f;
             fieldsDirty_ = true;
             fragValid_ = false;
-            if (f == fieldBF16x2 && d_J_.ptr) {
+            if (d_J_.ptr) {
                 ensureDevice();
-                d_Jsplit_.allocate((size_t)N_ * Np_);
-                splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jsplit_.ptr, dev_->stream());
+                prepareFieldForm();
             }
         }
     } else if (name == "refresh_engine") {
@@ -346,7 +348,9 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("device", "rocm");
     p.emplace_back("rng", rngMT_ ? "mt19937" : "philox");
     p.emplace_back("sweep", plainSweep_ ? "plain" : "local");
-    p.emplace_back("field_precision", fieldFormat_ == fieldBF16x2 ? "bf16x2" : (sizeof(real) == 4 ? "fp32" : "fp64"));
+    p.emplace_back("field_precision", fieldFormat_ == fieldBF16x2 ? "bf16x2"
+                                      : fieldFormat_ == fieldFP8  ? "fp8"
+                                                                  : (sizeof(real) == 4 ? "fp32" : "fp64"));
     p.emplace_back("slices_per_block", std::to_string(slicesPerBlock_));
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
     p.emplace_back("refresh_engine", refreshVector_ ? "vector" : "mfma");
@@ -372,15 +376,37 @@ void DenseGraphAnnealer<real>::uploadJ(const real *J, const real *h, real c) {
     HIP_CHECK(hipMemcpy2DAsync(d_J_.ptr, (size_t)Np_ * sizeof(real), J, (size_t)N_ * sizeof(real),
                                (size_t)N_ * sizeof(real), N_, hipMemcpyHostToDevice, dev_->stream()));
     HIP_CHECK(hipMemcpyAsync(d_h_.ptr, h, (size_t)N_ * sizeof(real), hipMemcpyHostToDevice, dev_->stream()));
-    if (fieldFormat_ == fieldBF16x2) {
-        d_Jsplit_.allocate((size_t)N_ * Np_);
-        splitBF16((const float *)d_J_.ptr, (size_t)N_ * Np_, d_Jsplit_.ptr, dev_->stream());
-    }
+    prepareFieldForm();
     fragValid_ = false;
     c_ = c;
     h_.assign(h, h + N_);
     dev_->synchronize();
     fieldsDirty_ = true;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::prepareFieldForm() {
+    if (fieldFormat_ == fieldF32) {
+        d_Jsplit_.release();
+        d_J8_.release();
+        d_scale_.release();
+        d_rowExp_.release();
+        return;
+    }
+    const float *J = (const float *)d_J_.ptr;
+    d_Jsplit_.allocate((size_t)N_ * Np_);
+    if (fieldFormat_ == fieldBF16x2) {
+        splitBF16(J, (size_t)N_ * Np_, d_Jsplit_.ptr, dev_->stream());
+        d_J8_.release();
+        d_scale_.release();
+        d_rowExp_.release();
+    } else {
+        ld8_ = fp8RowStride(Np_);
+        d_J8_.allocate((size_t)N_ * ld8_);
+        d_scale_.allocate((size_t)Np_);
+        d_rowExp_.allocate((size_t)N_);
+        quantizeFP8(J, N_, Np_, ld8_, d_rowExp_.ptr, d_scale_.ptr, d_J8_.ptr, d_Jsplit_.ptr, dev_->stream());
+    }
 }
 
 template <class real>
@@ -575,18 +601,29 @@ void DenseGraphAnnealer<real>::refreshFields() {
     if (mfma && !fragValid_) {
         /* J in MFMA fragment order, built once per problem and form */
         const float *J = (const float *)d_J_.ptr;
+        d_frag16_.release();
+        d_frag32_.release();
+        d_frag8_.release();
         if (fieldFormat_ == fieldBF16x2) {
-            d_frag32_.release();
             d_frag16_.allocate(fieldFragElemsBF16x2(Np_, Np_));
             buildFragBF16x2(J, N_, Np_, Np_, Np_, d_frag16_.ptr, d_frag16_.n, dev_->stream());
+        } else if (fieldFormat_ == fieldFP8) {
+            d_frag8_.allocate(fieldFragElemsFP8(Np_, Np_));
+            buildFragFP8(d_J8_.ptr, N_, ld8_, Np_, Np_, d_frag8_.ptr, d_frag8_.n, dev_->stream());
         } else {
-            d_frag16_.release();
             d_frag32_.allocate(fieldFragElemsF32(Np_, Np_));
             buildFragF32(J, N_, Np_, Np_, Np_, d_frag32_.ptr, d_frag32_.n, dev_->stream());
         }
         fragValid_ = true;
     }
-    if (fieldFormat_ == fieldBF16x2) {
+    if (fieldFormat_ == fieldFP8) {
+        /* the vector cross-check runs on J' as floats */
+        if (mfma)
+            fieldRefreshFP8(d_frag8_.ptr, d_frag8_.n, d_scale_.ptr, d_Q_.ptr, Np_, Np_, m_, Nw_, Np_,
+                            (float *)d_F_.ptr, dev_->stream());
+        else
+            fieldRefresh<float>(d_Jsplit_.ptr, N_, Np_, Np_, d_Q_.ptr, m_, Nw_, Np_, (float *)d_F_.ptr, dev_->stream());
+    } else if (fieldFormat_ == fieldBF16x2) {
         /* the vector cross-check runs on Jsplit = float(hi) + float(lo), the
          * same numbers as the two planes */
         if (mfma)
@@ -649,7 +686,10 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     a.coef = std::log(std::tanh(G * beta / real(m_))) / beta;
     a.beta = beta;
     a.invKT = real(1.) / G;
-    a.J = (fieldFormat_ == fieldBF16x2 && !plainSweep_) ? reinterpret_cast<const real *>(d_Jsplit_.ptr) : d_J_.ptr;
+    a.J = (fieldFormat_ != fieldF32 && !plainSweep_) ? reinterpret_cast<const real *>(d_Jsplit_.ptr) : d_J_.ptr;
+    a.J8 = d_J8_.ptr;
+    a.rowScale = d_scale_.ptr;
+    a.ld8 = ld8_;
     a.h = d_h_.ptr;
     a.F = d_F_.ptr;
     a.Qbits = d_Q_.ptr;
@@ -670,12 +710,14 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     int S = slicesPerBlock_;
     if (S <= 0) {
         S = 4;
-        while (S > 1 && (denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_) > dev_->maxLdsPerBlock() || S > m_))
+        while (S > 1 && (denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) >
+                             dev_->maxLdsPerBlock() ||
+                         S > m_))
             S >>= 1;
     }
-    SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_) > dev_->maxLdsPerBlock(),
+    SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) > dev_->maxLdsPerBlock(),
                 "N=%d too large for the LDS-resident sweep (%zu bytes needed per block)", N_,
-                denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_));
+                denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_));
     if (profile_)
         phaseBegin(2, pe);
     denseSweep<real>(a, plainSweep_, fieldFormat_, S, dev_->stream());
@@ -710,6 +752,34 @@ void DenseGraphAnnealer<real>::calculate_E() {
             lastERoundErr_ = err;
     }
     setState(solEAvailable);
+}
+
+template <class real>
+double DenseGraphAnnealer<real>::formEnergyError() {
+    throwErrorIfQNotSet();
+    ensureDevice();
+    if constexpr (sizeof(real) != 4) {
+        return 0.;
+    } else {
+        if (fieldFormat_ == fieldF32)
+            return 0.;
+        std::vector<double> e((size_t)m_), ef((size_t)m_);
+        energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
+                         dev_->stream());
+        HIP_CHECK(hipMemcpyAsync(e.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
+                                 dev_->stream()));
+        dev_->synchronize();
+        energyFP64<real>(d_Jsplit_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
+                         dev_->stream());
+        HIP_CHECK(hipMemcpyAsync(ef.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
+                                 dev_->stream()));
+        dev_->synchronize();
+        checkAbort(true);
+        double err = 0.;
+        for (int y = 0; y < m_; ++y)
+            err = std::max(err, std::fabs(ef[y] - e[y]));
+        return err;
+    }
 }
 
 template <class real>

@@ -9,6 +9,9 @@
  *  - bf16x2: J = hi + lo (two bf16 planes), spins +-1 exact in bf16,
  *    v_mfma_f32_16x16x32_bf16 twice per k-block, fp32 accumulation.
  *  - fp32:   v_mfma_f32_16x16x4_f32, spins as +-1.0f.
+ *  - fp8:    J as e4m3 bytes with a power-of-two scale per column, spins
+ *    +-1 exact in e4m3, v_mfma_scale_f32_16x16x128_f8f6f4 with unit block
+ *    scales; the column scale multiplies the reduced sum (exactly).
  *
  * J is read from a copy laid out in MFMA fragment order, built once per
  * problem: for column block cb (16 columns) and k-block kb, the 64 lanes'
@@ -62,8 +65,10 @@ __device__ __forceinline__ bf16x8 spinsBF16(uint32_t bits) {
 }
 
 /* Fixed-order sum of the kSplitK partial tiles in LDS and the store of the
- * pass's rows: red[w][4 t + r][lane] holds wave w's acc[t][r]. */
-__device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int y0, int m, int cb, int ldF, float *F) {
+ * pass's rows: red[w][4 t + r][lane] holds wave w's acc[t][r]; colScale (if
+ * given) multiplies each column's sum. */
+__device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int y0, int m, int cb, int ldF, float *F,
+                                            const float *colScale = nullptr) {
     __syncthreads();
 #pragma unroll 4
     for (int j = 0; j < kTiles * 4 * kWave / kBlockThreads; ++j) {
@@ -75,6 +80,8 @@ __device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int
 #pragma unroll
             for (int w = 1; w < kSplitK; ++w)
                 s += red[w][tr][L];
+            if (colScale)
+                s *= colScale[cb * 16 + (L & 15)];
             F[(long long)y * ldF + cb * 16 + (L & 15)] = s;
         }
     }
@@ -86,6 +93,10 @@ __device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int
 __global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const uint4 *frag, const uint32_t *Qbits,
                                                                           int K, int m, int Nw, int ldF, float *F) {
     __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
+    __shared__ uint4 lut[256];                        /* byte -> 8 spins as bf16 +-1 (4 KiB) */
+    for (int i = threadIdx.x; i < 256; i += kBlockThreads)
+        lut[i] = __builtin_bit_cast(uint4, spinsBF16((uint32_t)i));
+    __syncthreads();
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
@@ -131,7 +142,7 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const 
                         const uint4 q4 = qw[buf][t >> 2];
                         const uint32_t qd = d == 0 ? q4.x : d == 1 ? q4.y : d == 2 ? q4.z : q4.w;
                         const uint32_t word = (uint32_t)__shfl((int)qd, 16 * (t & 3) + c, kWave);
-                        const bf16x8 a = spinsBF16((word >> (8 * g)) & 0xffu);
+                        const bf16x8 a = __builtin_bit_cast(bf16x8, lut[(word >> (8 * g)) & 0xffu]);
                         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bhi, acc[t], 0, 0, 0);
                         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, blo, acc[t], 0, 0, 0);
                     }
@@ -237,6 +248,131 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshF32Kernel(const flo
     }
 }
 
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+constexpr int kDepth8 = 2; /* fp8: k-blocks of 128 per super-block (256 k = 8 spin words) */
+constexpr int kE4M3Unit = 127; /* E8M0 block scale 2^0 */
+
+/* 8 spins (bits of `bits`) as e4m3 +-1: 0x38 = +1, 0xB8 = -1 */
+__device__ __forceinline__ uint2 spinsFP8(uint32_t bits) {
+    uint32_t v[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const uint32_t nib = (bits >> (4 * j)) & 15u;
+        v[j] = 0xB8B8B8B8u ^ (((nib * 0x204081u) & 0x01010101u) << 7);
+    }
+    return uint2{v[0], v[1]};
+}
+
+/* frag: [column blocks][K/128 k-blocks][2 halves][64 lanes] x 16 B.  In
+ * super-block s (k-blocks 2s, 2s + 1), lane group g of k-block 2s + d holds
+ * k = 256 s + 64 g + 32 d + e, e < 32: spin word 8 s + 2 g + d of the row,
+ * bit e -- so each lane's A operand for a whole super-block is the 8-byte
+ * pair of words (8 s + 2 g, 8 s + 2 g + 1) of its row, loaded directly. */
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshFP8Kernel(const uint4 *frag, const float *colScale,
+                                                                       const uint32_t *Qbits, int K, int m, int Nw,
+                                                                       int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
+    __shared__ uint2 lut[256];                        /* byte -> 8 spins as e4m3 +-1 (2 KiB) */
+    for (int i = threadIdx.x; i < 256; i += kBlockThreads)
+        lut[i] = spinsFP8((uint32_t)i);
+    __syncthreads();
+    const int lane = threadIdx.x % kWave;
+    const int w = threadIdx.x / kWave;
+    const int c = lane & 15, g = lane >> 4;
+    const int cb = blockIdx.x;
+    const int nK = K / 128;
+    const int nS = nK / kDepth8;
+    const int sOff = (int)(((long long)cb * 3) % nS);
+    const uint4 *fb = frag + (size_t)cb * nK * 2 * kWave + lane;
+    for (int y0 = 0; y0 < m; y0 += kRowsPerPass) {
+        f32x4 acc[kTiles];
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t)
+            acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        uint4 b0[2][kDepth8], b1[2][kDepth8];
+        uint2 qv[2][kTiles];
+        auto fetch = [&](int buf, int l) {
+            int s = l + sOff;
+            if (s >= nS)
+                s -= nS;
+#pragma unroll
+            for (int d = 0; d < kDepth8; ++d) {
+                const uint4 *p = fb + (size_t)(s * kDepth8 + d) * 2 * kWave;
+                b0[buf][d] = p[0];
+                b1[buf][d] = p[kWave];
+            }
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) {
+                const int y = y0 + 16 * t + c;
+                qv[buf][t] = (y0 + 16 * t < m && y < m)
+                                 ? *reinterpret_cast<const uint2 *>(Qbits + (size_t)y * Nw + 8 * s + 2 * g)
+                                 : uint2{0u, 0u};
+            }
+        };
+        auto compute = [&](int buf) {
+#pragma unroll
+            for (int d = 0; d < kDepth8; ++d) {
+                const uint4 x0 = b0[buf][d], x1 = b1[buf][d];
+                const i32x8 bop = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w,
+                                   (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+#pragma unroll
+                for (int t = 0; t < kTiles; ++t) {
+                    if (y0 + 16 * t < m) { /* wave-uniform */
+                        const uint32_t word = d == 0 ? qv[buf][t].x : qv[buf][t].y;
+                        const uint2 l0 = lut[word & 0xffu], l1 = lut[(word >> 8) & 0xffu];
+                        const uint2 l2 = lut[(word >> 16) & 0xffu], l3 = lut[word >> 24];
+                        const i32x8 aop = {(int)l0.x, (int)l0.y, (int)l1.x, (int)l1.y,
+                                           (int)l2.x, (int)l2.y, (int)l3.x, (int)l3.y};
+                        acc[t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aop, bop, acc[t], 0, 0, 0,
+                                                                                  kE4M3Unit, 0, kE4M3Unit);
+                    }
+                }
+            }
+        };
+        for (int l = w; l < nS; l += 2 * kSplitK) {
+            if (l == w)
+                fetch(0, l);
+            if (l + kSplitK < nS)
+                fetch(1, l + kSplitK);
+            compute(0);
+            if (l + kSplitK >= nS)
+                break;
+            if (l + 2 * kSplitK < nS)
+                fetch(0, l + 2 * kSplitK);
+            compute(1);
+        }
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                red[w][4 * t + r][lane] = acc[t][r];
+        reduceStore(red, y0, m, cb, ldF, F, colScale);
+    }
+}
+
+/* One thread per (column block, k-block, lane): the 32 bytes of the lane's
+ * column (a row of B8, stride ld8, zero past `rows`) in the order above. */
+__global__ void buildFragFP8Kernel(const uint8_t *B8, int rows, int ld8, int nColBlocks, int K, uint4 *frag) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nK = K / 128;
+    if (idx >= (size_t)nColBlocks * nK * kWave)
+        return;
+    const int lane = (int)(idx % kWave);
+    const size_t rest = idx / kWave;
+    const int kb = (int)(rest % nK), cb = (int)(rest / nK);
+    const int col = cb * 16 + (lane & 15);
+    const int k0 = 256 * (kb / kDepth8) + 64 * (lane >> 4) + 32 * (kb % kDepth8);
+    uint4 v0 = {0u, 0u, 0u, 0u}, v1 = {0u, 0u, 0u, 0u};
+    if (col < rows) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(B8 + (size_t)col * ld8 + k0);
+        v0 = src[0];
+        v1 = src[1];
+    }
+    uint4 *dst = frag + ((size_t)cb * nK + kb) * 2 * kWave + lane;
+    dst[0] = v0;
+    dst[kWave] = v1;
+}
+
 /* One thread per (column block, k-block, lane): the lane's 8 k values of its
  * column (a row of B, row stride ld, zero past `rows`), split into bf16 hi and lo. */
 __global__ void buildFragBF16x2Kernel(const float *B, int rows, int ld, int nColBlocks, int K, uint4 *frag) {
@@ -289,6 +425,31 @@ void checkGemm(int nCols, int K, int ld, int Nw) {
 
 size_t fieldFragElemsBF16x2(int nCols, int K) { return 2 * (size_t)nCols * K; } /* hi and lo planes, uint16 each */
 size_t fieldFragElemsF32(int nCols, int K) { return (size_t)nCols * K; }
+size_t fieldFragElemsFP8(int nCols, int K) { return (size_t)nCols * K; } /* bytes */
+
+void buildFragFP8(const uint8_t *d_B8, int rows, int ld8, int nCols, int K, uint8_t *d_frag, size_t fragElems,
+                  hipStream_t stream) {
+    checkGemm(nCols, K, ld8, K / 32);
+    SQ_THROW_IF(ld8 % 16 != 0, "fp8 fragments: row stride %d is not a multiple of 16 bytes", ld8);
+    SQ_THROW_IF(rows > nCols, "field GEMM: %d rows exceed %d columns", rows, nCols);
+    SQ_THROW_IF(fragElems < fieldFragElemsFP8(nCols, K), "fp8 fragment buffer holds %zu of %zu bytes", fragElems,
+                fieldFragElemsFP8(nCols, K));
+    const size_t threads = (size_t)(nCols / 16) * (K / 128) * kWave;
+    buildFragFP8Kernel<<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(d_B8, rows, ld8, nCols / 16, K,
+                                                                              reinterpret_cast<uint4 *>(d_frag));
+    HIP_CHECK(hipGetLastError());
+}
+
+void fieldRefreshFP8(const uint8_t *d_frag, size_t fragElems, const float *d_colScale, const uint32_t *d_Qbits,
+                     int nCols, int K, int m, int Nw, int ldF, float *d_F, hipStream_t stream) {
+    checkGemm(nCols, K, K, Nw);
+    SQ_THROW_IF(ldF < nCols, "field GEMM: output stride %d < %d columns", ldF, nCols);
+    SQ_THROW_IF(fragElems < fieldFragElemsFP8(nCols, K), "fp8 fragment buffer too small");
+    SQ_THROW_IF(!d_colScale, "fp8 field GEMM: no column scales");
+    fieldRefreshFP8Kernel<<<nCols / 16, kBlockThreads, 0, stream>>>(reinterpret_cast<const uint4 *>(d_frag),
+                                                                    d_colScale, d_Qbits, K, m, Nw, ldF, d_F);
+    HIP_CHECK(hipGetLastError());
+}
 
 void buildFragBF16x2(const float *d_B, int rows, int ld, int nCols, int K, uint16_t *d_frag, size_t fragElems,
                      hipStream_t stream) {

@@ -141,10 +141,25 @@ struct Expo<double> {
     __device__ static double eval(double v, const uint64_t *) { return exp(v); }
 };
 
-template <class real, bool PLAIN>
+/* Kernel variants: local fields with rows of args.J, the plain reference
+ * algorithm, and local fields with the fp8 rows of args.J8 (float only). */
+enum SweepMode { kModeLocal = 0, kModePlain = 1, kModeFP8 = 2 };
+
+/* 4 e4m3 bytes -> floats (exact), gfx950's OCP FP8 conversion */
+__device__ __forceinline__ void fp8x4ToFloat(uint32_t w, float v[4]) {
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    const f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w, false);
+    const f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w, true);
+    v[0] = lo.x; v[1] = lo.y; v[2] = hi.x; v[3] = hi.y;
+}
+
+template <class real, int MODE>
 __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
                                                                                 int xcdRemap) {
 #pragma clang fp contract(off)
+    constexpr bool PLAIN = MODE == kModePlain;
+    constexpr bool FP8 = MODE == kModeFP8;
+    static_assert(!FP8 || sizeof(real) == 4, "fp8 rows are for float solvers");
     extern __shared__ __align__(16) unsigned char lds[];
     const int N = a.N, Np = a.Np, m = a.m, Nw = a.Nw;
     const int lane = threadIdx.x & (kWave - 1);
@@ -165,10 +180,15 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
     real *hS = reinterpret_cast<real *>(lds + kExpTabBytes);
     unsigned char *p = lds + kExpTabBytes + (size_t)Np * sizeof(real);
-    const size_t perWave = (PLAIN ? 0 : (size_t)Np * sizeof(real)) + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
+    constexpr bool kDefer = !PLAIN && sizeof(real) == 4; /* deferred row updates, staged in LDS */
+    const size_t stageBytes = kDefer ? (FP8 ? (size_t)a.ld8 : (size_t)Np * sizeof(real)) : 0;
+    const size_t fieldBytes = PLAIN ? 0 : (size_t)Np * sizeof(real) + stageBytes;
+    const size_t perWave = fieldBytes + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
     p += perWave * w;
     real *F = reinterpret_cast<real *>(p);
-    int *tbl = reinterpret_cast<int *>(p + (PLAIN ? 0 : (size_t)Np * sizeof(real)));
+    real *stage = F + Np; /* kDefer: the J row of the last flip, as loaded */
+    const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(stage); /* FP8: the e4m3 row */
+    int *tbl = reinterpret_cast<int *>(p + fieldBytes);
     uint32_t *Qb = reinterpret_cast<uint32_t *>(tbl + tblN);
     uint32_t *Mir[2] = {Qb + Nw, Qb + 2 * Nw};
 
@@ -205,6 +225,29 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     unsigned long long nFlips = 0, nBatches = 0, nBlocked = 0;
     int spins = 0;
     bool aborted = false;
+    /* kDefer: the J row of the last committed flip is loaded straight into
+     * `stage` (global_load_lds) and added to F at the next commit; meanwhile
+     * evaluations add rowD * stage[s] to F[s] themselves -- the same float
+     * operation as the LDS update, so fields are bit-identical.  FP8: the row
+     * is e4m3 and rowD = -+2 2^a_x, so rowD * byte = -+2 J'[x][s] exactly, the
+     * float the fp32 form adds for the same J'. */
+    bool rowPending = false;
+    real rowD = 0;
+    auto applyPendingRow = [&]() {
+        for (int j = lane * 4; j < Np; j += kWave * 4) {
+            real fv[4], sv[4];
+            ldsLoad4(F + j, fv);
+            if constexpr (FP8)
+                fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(stage8 + j), sv);
+            else
+                ldsLoad4(stage + j, sv);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                fv[t] = fv[t] + rowD * sv[t];
+            ldsStore4(F + j, fv);
+        }
+        rowPending = false;
+    };
 
     /* Own attempt stream: a rolling window of 128 entries in two halves
      * (entries wk + lane and wk + 64 + lane); a batch at k < wk + 64 reads
@@ -268,6 +311,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
 
         /* neighbours: replay decided prefixes, register pending entries */
         bool pendSet[2][2] = {{false, false}, {false, false}};
+        bool pendShared[2][2] = {{false, false}, {false, false}}; /* the slot already held an entry */
         if (!a.sa) {
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
@@ -309,9 +353,9 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
                 pendSet[nb][0] = e0 >= A && e0 < N && e0 - Kk < kWave - 1;
                 pendSet[nb][1] = e1 >= A && e1 < N && e1 - Kk < kWave - 1;
                 if (pendSet[nb][0])
-                    atomicMin(&tbl[ns[nb][0] & tblMask], pendCode(e0 - Kk, N, nb));
+                    pendShared[nb][0] = atomicMin(&tbl[ns[nb][0] & tblMask], pendCode(e0 - Kk, N, nb)) != kPendEmpty;
                 if (pendSet[nb][1])
-                    atomicMin(&tbl[ns[nb][1] & tblMask], pendCode(e1 - Kk, N, nb));
+                    pendShared[nb][1] = atomicMin(&tbl[ns[nb][1] & tblMask], pendCode(e1 - Kk, N, nb)) != kPendEmpty;
                 /* lane i needs entries [A, Kk + i): they must lie in the window */
                 const int winEnd = base + 2 * kWave < N ? base + 2 * kWave : N;
                 if (winEnd < N) {
@@ -321,24 +365,37 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             }
         }
         waveSync();
-        /* a slot written by more than one entry is marked: lanes meeting it
-         * wait instead of deciding under both outcomes of one entry */
+        /* a slot written by more than one entry is marked (every registration
+         * after the first found it occupied): lanes meeting it wait instead
+         * of deciding under both outcomes of one entry */
         int curBase[2] = {nbase[0], nbase[1]};
         if (!a.sa) {
+            const bool anyShared = pendShared[0][0] || pendShared[0][1] || pendShared[1][0] || pendShared[1][1];
+            if (__ballot(anyShared)) {
 #pragma unroll
-            for (int nb = 0; nb < 2; ++nb) {
-                const int Kk = k + delta[nb];
+                for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int e = curBase[nb] + h * kWave + lane;
-                    if (pendSet[nb][h]) {
-                        int *slot = &tbl[ns[nb][h] & tblMask];
-                        if (*slot != pendCode(e - Kk, N, nb))
-                            atomicOr(slot, kPendMulti);
-                    }
+                    for (int h = 0; h < 2; ++h)
+                        if (pendShared[nb][h])
+                            atomicOr(&tbl[ns[nb][h] & tblMask], kPendMulti);
+                waveSync();
+            }
+        }
+        /* the staged row has had the whole neighbour phase to arrive: wait for
+         * it before the next windows are requested (vmcnt counts in order) */
+        real rowCorr = 0;
+        if constexpr (kDefer) {
+            if (rowPending) {
+                __builtin_amdgcn_s_waitcnt(0);
+                waveSync();
+                if constexpr (FP8) {
+                    float v4[4];
+                    fp8x4ToFloat(stage8[s], v4);
+                    rowCorr = v4[0];
+                } else {
+                    rowCorr = stage[s];
                 }
             }
-            waveSync();
         }
         /* the next batch's neighbour windows start where this batch's replay
          * ended: fetch them now, so the loads overlap evaluation and commit */
@@ -397,7 +454,10 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
         } else {
             /* decided under both outcomes of the pending attempt: if they
              * agree the decision is final, otherwise the lane waits */
-            const real f = F[s];
+            real f = F[s];
+            if constexpr (kDefer)
+                if (rowPending)
+                    f = f + rowD * rowCorr;
             accLocal = decide(f);
             if (hypNb >= 0 && decideWith(f, nsumAlt) != accLocal)
                 blocked = true;
@@ -534,13 +594,38 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
             storeShared32(a.D + yN + k + lane, (tag << 1) | (lane == flipLane ? 1u : 0u));
 
         SQ_CYC(c4);
+        if constexpr (kDefer)
+            if (rowPending)
+                applyPendingRow();
         if (flipLane >= 0) {
             const uint32_t x = __shfl(s, flipLane);
             const bool qx = __shfl((int)qpos, flipLane) != 0;
             ++nFlips;
             if (lane == 0)
                 atomicXor(&Qb[x >> 5], 1u << (x & 31));
-            if constexpr (!PLAIN) {
+            if constexpr (FP8) {
+                /* the same deferred update from the e4m3 row (ld8 bytes, a
+                 * multiple of 1 KiB) and its scale */
+                rowD = (qx ? -2.f : 2.f) * a.rowScale[x];
+                const uint8_t *row = a.J8 + (long long)x * a.ld8;
+                waveSync();
+                for (int j0 = 0; j0 < a.ld8; j0 += kWave * 16)
+                    __builtin_amdgcn_global_load_lds((const void *)(row + j0 + lane * 16),
+                                                     (__attribute__((address_space(3))) void *)(stage + j0 / 4), 16,
+                                                     0, 0);
+                rowPending = true;
+            } else if constexpr (kDefer) {
+                /* F[y][:] += -2 q_x J[x][:] (J symmetric), deferred: the row
+                 * streams into `stage`, 1 KiB (16 B per lane) per load */
+                rowD = qx ? (real)-2. : (real)2.;
+                const real *row = a.J + (long long)x * Np;
+                waveSync(); /* this wave's reads of `stage` are done */
+                for (int j0 = 0; j0 < Np; j0 += kWave * 4)
+                    __builtin_amdgcn_global_load_lds((const void *)(row + j0 + lane * 4),
+                                                     (__attribute__((address_space(3))) void *)(stage + j0), 16, 0,
+                                                     0);
+                rowPending = true;
+            } else if constexpr (!PLAIN) {
                 /* F[y][:] += -2 q_x J[:][x] = -2 q_x J[x][:]  (J symmetric);
                  * the row's loads go out together, then the LDS updates */
                 const real d = qx ? (real)-2. : (real)2.;
@@ -576,6 +661,14 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
         SQ_CYC_ADD(cyc[4], c4, c5);
     }
 
+    if constexpr (kDefer) {
+        if (rowPending && !aborted) {
+            __builtin_amdgcn_s_waitcnt(0);
+            waveSync();
+            applyPendingRow();
+            waveSync();
+        }
+    }
     if (!aborted) {
         if (!PLAIN)
             for (int i = lane; i < Np; i += kWave)
@@ -594,12 +687,35 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     }
 }
 
+__global__ void fp8DecodeKernel(float *out) {
+    const int t = threadIdx.x; /* 64 threads, 4 codes each */
+    float v[4];
+    fp8x4ToFloat((uint32_t)(4 * t) | ((uint32_t)(4 * t + 1) << 8) | ((uint32_t)(4 * t + 2) << 16) |
+                     ((uint32_t)(4 * t + 3) << 24),
+                 v);
+    for (int i = 0; i < 4; ++i)
+        out[4 * t + i] = v[i];
+}
+
 }  // namespace
 
-size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain) {
+void fp8DecodeTable(float out[256]) {
+    float *d = nullptr;
+    HIP_CHECK(hipMalloc((void **)&d, 256 * sizeof(float)));
+    fp8DecodeKernel<<<1, 64>>>(d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpy(out, d, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIP_CHECK(e);
+}
+
+size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, int ld8) {
     int tblN = pendingTableSize(Np);
     int Nw = Np / 32;
-    size_t perWave = (plain ? 0 : (size_t)Np * realSize) + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
+    const bool defer = !plain && realSize == 4;
+    const size_t stage = defer ? (fieldFormat == fieldFP8 ? (size_t)ld8 : (size_t)Np * realSize) : 0;
+    size_t perWave = (plain ? 0 : (size_t)Np * realSize + stage) + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
     return kExpTabBytes + (size_t)Np * realSize + perWave * slicesPerBlock;
 }
 
@@ -608,12 +724,17 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
     SQ_THROW_IF(args.Np % 256 != 0, "padded N must be a multiple of 256");
     SQ_THROW_IF(slicesPerBlock < 1 || slicesPerBlock > kMaxSlicesPerBlock, "slices per block out of range (1..%d)",
                 kMaxSlicesPerBlock);
+    const bool fp8 = !plain && fieldFormat == fieldFP8;
+    SQ_THROW_IF(fp8 && (sizeof(real) != 4 || !args.J8 || !args.rowScale || args.ld8 < args.Np || args.ld8 % 1024),
+                "fp8 sweep: needs a float solver and e4m3 rows with a 1 KiB-multiple stride");
     int nBlocks = divru(args.m, slicesPerBlock);
-    size_t lds = denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain);
+    size_t lds = denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain, fieldFormat, args.ld8);
     int xcdRemap = (nBlocks % 8 == 0 && nBlocks >= 16) ? 1 : 0;
 
-    (void)fieldFormat; /* the J rows passed in already carry the field format */
-    void *fn = plain ? (void *)denseSweepKernel<real, true> : (void *)denseSweepKernel<real, false>;
+    void *fn = plain ? (void *)denseSweepKernel<real, kModePlain> : (void *)denseSweepKernel<real, kModeLocal>;
+    if constexpr (sizeof(real) == 4)
+        if (fp8)
+            fn = (void *)denseSweepKernel<real, kModeFP8>;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SweepArgs<real> a = args;
     void *kargs[] = {(void *)&a, (void *)&slicesPerBlock, (void *)&xcdRemap};

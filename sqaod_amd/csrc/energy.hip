@@ -161,3 +161,63 @@ template void energyBipartiteFP64<double>(const double *, const double *, const 
                                           const uint32_t *, int, int, int, int, int, double *, double *, hipStream_t);
 
 }  // namespace sqaod_rocm
+
+/* ---------------- batched bilinear forms (the formulas objects) ---------------- */
+
+namespace sqaod_rocm {
+
+namespace {
+
+constexpr int kQfThreads = 256;
+
+/* One block per output: E[o] = cst + sgn (sum_j lin0[j] v[j] + sum_i u[i] (lin1[i] + sum_j A[i][j] v[j])),
+ * u = U[iu[o]] (nu entries), v = V[iv[o]] (nv entries), int8 vectors; FP64
+ * accumulation, fixed reduction order (per-thread strided sums, then a tree). */
+template <class real>
+__global__ __launch_bounds__(kQfThreads) void quadFormKernel(QuadFormArgs<real> a) {
+    __shared__ double red[kQfThreads];
+    const int o = blockIdx.x;
+    const int bu = a.twoD ? o / a.nV : o;
+    const int bv = a.twoD ? o % a.nV : o;
+    const int8_t *u = a.U + (size_t)bu * a.nu;
+    const int8_t *v = a.V + (size_t)bv * a.nv;
+    double s = 0.;
+    for (int i = threadIdx.x; i < a.nu; i += kQfThreads) {
+        if (u[i] == 0)
+            continue;
+        double r = a.lin1 ? (double)a.lin1[i] : 0.;
+        const real *Ai = a.A + (size_t)i * a.nv;
+        for (int j = 0; j < a.nv; ++j)
+            if (v[j])
+                r += (double)Ai[j] * (double)v[j];
+        s += (double)u[i] * r;
+    }
+    if (a.lin0)
+        for (int j = threadIdx.x; j < a.nv; j += kQfThreads)
+            s += (double)a.lin0[j] * (double)v[j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kQfThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        a.E[o] = (real)(a.cst + a.sgn * red[0]);
+}
+
+}  // namespace
+
+template <class real>
+void quadForm(const QuadFormArgs<real> &a, int nOut, hipStream_t stream) {
+    if (nOut <= 0)
+        return;
+    SQ_THROW_IF(a.nu <= 0 || a.nv <= 0, "bilinear form: empty vectors");
+    quadFormKernel<real><<<nOut, kQfThreads, 0, stream>>>(a);
+    HIP_CHECK(hipGetLastError());
+}
+
+template void quadForm<float>(const QuadFormArgs<float> &, int, hipStream_t);
+template void quadForm<double>(const QuadFormArgs<double> &, int, hipStream_t);
+
+}  // namespace sqaod_rocm

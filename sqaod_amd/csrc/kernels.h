@@ -63,14 +63,25 @@ void unpackSpins(const uint32_t *d_Qbits, int N, int m, int Nw, int8_t *d_q, hip
 /* ---------------- dense local fields (dense_field.hip) ---------------- */
 
 enum FieldFormat {
-    fieldF32 = 0,   /* J stored as real */
-    fieldBF16x2 = 1 /* J ~ hi + lo, both bf16, hi = bf16(J), lo = bf16(J - hi) */
+    fieldF32 = 0,    /* J stored as real */
+    fieldBF16x2 = 1, /* J ~ hi + lo, both bf16, hi = bf16(J), lo = bf16(J - hi) */
+    fieldFP8 = 2     /* J ~ J' = 2^a_i e4m3 per row i (quantizeFP8) */
 };
 
 /* Jsplit = float(bf16(J)) + float(bf16(J - bf16(J))) elementwise (exact in
  * fp32): the J the bf16x2 sweep's row updates use, so that incremental fields
  * stay consistent with the split GEMM. */
 void splitBF16(const float *d_J, size_t count, float *d_Jsplit, hipStream_t stream);
+
+/* FP8 form of a symmetric J (N x ld floats): each row i gets a power-of-two
+ * scale 2^a_i (the smallest with max_j |J[i][j]| <= 448 2^a_i, raised to
+ * within 15 binades of the largest), and J'[i][j] is J[i][j] rounded to
+ * nearest-even on the e4m3 grid of 2^max(a_i, a_j): J' is symmetric, and
+ * J'[i][j] / 2^a_i is an exact e4m3 value in both rows.  Outputs: d_rowExp
+ * (N ints), d_scale (ld floats, 1 past N), d_J8 (N x ld8 bytes, e4m3
+ * J'/2^a_i, zero past N) and d_Jq = J' (N x ld floats). */
+void quantizeFP8(const float *d_J, int N, int ld, int ld8, int *d_rowExp, float *d_scale, uint8_t *d_J8, float *d_Jq,
+                 hipStream_t stream);
 
 /* F[y][i] = sum_{j<K} q[y][j] * B[j][i] for y < m, i < nCols (B row-major with
  * row stride ldB >= nCols, F row stride ldF), spins from packed bits (Nw words
@@ -97,6 +108,15 @@ void fieldRefreshBF16x2(const uint16_t *d_frag, size_t fragElems, const uint32_t
                         int Nw, int ldF, float *d_F, hipStream_t stream);
 void fieldRefreshF32MFMA(const float *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
                          int Nw, int ldF, float *d_F, hipStream_t stream);
+/* FP8 form: B8 holds e4m3 bytes (rows x ld8) whose row c is scaled by
+ * colScale[c] (a power of two); F[y][c] = colScale[c] sum_k q[y][k] B8[c][k]
+ * on v_mfma_scale_f32_16x16x128_f8f6f4 (unit block scales), fp32
+ * accumulation.  fieldFragElemsFP8 = nCols K bytes. */
+size_t fieldFragElemsFP8(int nCols, int K);
+void buildFragFP8(const uint8_t *d_B8, int rows, int ld8, int nCols, int K, uint8_t *d_frag, size_t fragElems,
+                  hipStream_t stream);
+void fieldRefreshFP8(const uint8_t *d_frag, size_t fragElems, const float *d_colScale, const uint32_t *d_Qbits,
+                     int nCols, int K, int m, int Nw, int ldF, float *d_F, hipStream_t stream);
 
 /* ---------------- bipartite annealer flips (bipartite.hip) ---------------- */
 
@@ -161,6 +181,9 @@ struct SweepArgs {
     real twoDivM, coef, beta;
     real invKT;
     const real *J;          /* N x Np rows used by the sweep (J, or its bf16 hi+lo rounding) */
+    const uint8_t *J8;      /* fp8 form: N x ld8 e4m3 rows, row x scaled by rowScale[x] */
+    const float *rowScale;
+    int ld8;
     const real *h;          /* Np */
     real *F;                /* m x Np, in/out */
     uint32_t *Qbits;        /* m x Nw, in/out */
@@ -176,13 +199,23 @@ struct SweepArgs {
 
 /* One annealOneStep sweep: N attempts per slice with exact serial-coloring
  * semantics.  plain = true computes each attempt's field as a J-row . q-row
- * dot (the reference algorithm); plain = false keeps the local fields in LDS.
- * fieldFormat is informational: args.J already holds the rows of the form. */
+ * dot (the reference algorithm); plain = false keeps the local fields in LDS
+ * and updates them with the rows of args.J (fp32 / bf16x2 forms) or of
+ * args.J8 (fieldFormat == fieldFP8, float solvers). */
 template <class real>
 void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, hipStream_t stream);
 
-/* Bytes of LDS one block of the local-field sweep needs. */
-size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain);
+/* Bytes of LDS one block of the local-field sweep needs (ld8: the fp8 row
+ * stride, used when fieldFormat == fieldFP8). */
+size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat = fieldF32,
+                          int ld8 = 0);
+
+/* The sweep's e4m3 decode applied to the 256 byte codes (a self check). */
+void fp8DecodeTable(float out[256]);
+
+/* Row stride of the fp8 rows: Np rounded up to 1 KiB (whole 16-byte-per-lane
+ * wave loads). */
+inline int fp8RowStride(int Np) { return (Np + 1023) / 1024 * 1024; }
 
 /* ---------------- energies (energy.hip) ---------------- */
 
@@ -199,6 +232,23 @@ template <class real>
 void energyBipartiteFP64(const real *d_JT, const real *d_h0, const real *d_h1, double c, const uint32_t *d_Q0,
                          const uint32_t *d_Q1, int N0, int N1, int N0p, int N1p, int m, double *d_partial,
                          double *d_E, hipStream_t stream);
+
+/* Batched bilinear forms for the formulas objects (energy.hip):
+ * E[o] = cst + sgn (lin0 . v + sum_i u_i (lin1_i + sum_j A[i][j] v_j)) with u, v
+ * int8 rows (x in {0,1} or q in {-1,+1}) taken from U[iu] and V[iv]: iu = iv =
+ * o (twoD = 0) or iu = o / nV, iv = o % nV (twoD = 1, all pairs).  A is
+ * nu x nv row-major; lin0 / lin1 may be null.  FP64 accumulation, result
+ * rounded to real once. */
+template <class real>
+struct QuadFormArgs {
+    const real *A, *lin0, *lin1;
+    double cst, sgn; /* E = cst + sgn * (bilinear + linear terms) */
+    const int8_t *U, *V;
+    int nu, nv, nV, twoD;
+    real *E;
+};
+template <class real>
+void quadForm(const QuadFormArgs<real> &a, int nOut, hipStream_t stream);
 
 /* sum_y q_y . q_(y+1)%m (the Trotter coupling of getSystemE), in double. */
 void spinDotSum(const uint32_t *d_Qbits, int N, int m, int Nw, double *d_out, hipStream_t stream);

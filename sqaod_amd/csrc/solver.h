@@ -144,6 +144,10 @@ public:
     /* Max |E_real - E_fp64| of the last calculate_E (the rounding of the
      * FP64 recompute to real). */
     double lastEnergyRoundingError() const { return lastERoundErr_; }
+    /* Max over slices of |E_form(q_y) - E(q_y)| for the current spins, both
+     * in FP64: E_form uses the J the field form anneals (bf16 hi + lo, or the
+     * fp8 J'), E the J as given; 0 for the fp32/fp64 form. */
+    double formEnergyError();
 
 private:
     void ensureDevice();
@@ -152,6 +156,7 @@ private:
     void refreshFields();
     void generateAttempts();
     void uploadJ(const real *J, const real *h, real c);
+    void prepareFieldForm(); /* derives the form's J (bf16 split / fp8 quantization) from d_J_ */
 
     Device *dev_ = nullptr;
     int N_ = 0, Np_ = 0, Nw_ = 0, m_ = -1;
@@ -165,7 +170,7 @@ private:
     /* preferences beyond the reference's */
     bool rngMT_ = false;          /* "rng": "philox" (default) | "mt19937" (CPU-stream replay) */
     bool plainSweep_ = false;     /* "sweep": "local" (default) | "plain" */
-    int fieldFormat_ = fieldF32;  /* "field_precision": "fp32" | "bf16x2" */
+    int fieldFormat_ = fieldF32;  /* "field_precision": "fp32" | "bf16x2" | "fp8" */
     int slicesPerBlock_ = 0;      /* "slices_per_block": 0 = automatic */
     int refreshInterval_ = 1;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
     bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
@@ -192,9 +197,14 @@ private:
 
     /* device state */
     DevBuf<real> d_J_, d_h_, d_F_, d_U_;
-    DevBuf<float> d_Jsplit_;  /* bf16x2 form: float(hi) + float(lo) */
+    DevBuf<float> d_Jsplit_;  /* the form's J as floats: bf16x2 float(hi) + float(lo), fp8 J' */
     DevBuf<uint16_t> d_frag16_; /* bf16x2 MFMA refresh: J's hi/lo planes in fragment order */
     DevBuf<float> d_frag32_;    /* fp32 MFMA refresh: J in fragment order */
+    DevBuf<uint8_t> d_J8_;      /* fp8 form: e4m3 rows (N x ld8_), row i scaled by d_scale_[i] */
+    DevBuf<uint8_t> d_frag8_;   /* fp8 MFMA refresh: d_J8_ in fragment order */
+    DevBuf<float> d_scale_;     /* fp8 form: 2^a_i per row (Np, 1 past N) */
+    DevBuf<int> d_rowExp_;
+    int ld8_ = 0;
     DevBuf<uint32_t> d_Q_, d_X_, d_D_, d_words_;
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
@@ -335,6 +345,34 @@ private:
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
     DevBuf<int8_t> d_q8_;
+};
+
+/* ------------------------------------------------------------------ */
+/* Formulas (DenseGraphFormulas / BipartiteGraphFormulas)                */
+/* ------------------------------------------------------------------ */
+template <class real>
+class Formulas {
+public:
+    using real_type = real;
+    void assignDevice(Device &dev);
+    /* x, q: nBatch rows of N int8; E: nBatch */
+    void denseCalculateE(real *E, const real *W, int N, const int8_t *x, int nBatch);
+    void denseHamiltonian(real *h, real *J, real *c, const real *W, int N);
+    void denseCalculateEFromSpin(real *E, const real *h, const real *J, real c, const int8_t *q, int N, int nBatch);
+    /* W, J: N1 x N0; x0: nBatch0 x N0, x1: nBatch1 x N1; E: nBatch0 (pairs)
+     * or nBatch1 x nBatch0 (twoD) */
+    void bipartiteCalculateE(real *E, const real *b0, const real *b1, const real *W, int N0, int N1,
+                             const int8_t *x0, int nBatch0, const int8_t *x1, int nBatch1, bool twoD);
+    void bipartiteHamiltonian(real *h0, real *h1, real *J, real *c, const real *b0, const real *b1, const real *W,
+                              int N0, int N1);
+    void bipartiteCalculateEFromSpin(real *E, const real *h0, const real *h1, const real *J, real c, int N0, int N1,
+                                     const int8_t *q0, const int8_t *q1, int nBatch);
+
+private:
+    Device &device();
+    void run(QuadFormArgs<real> a, const real *A, size_t nA, const real *lin0, size_t n0, const real *lin1,
+             size_t n1, const int8_t *U, size_t nU, const int8_t *V, size_t nV, real *E, int nOut);
+    Device *dev_ = nullptr;
 };
 
 }  // namespace sqaod_rocm

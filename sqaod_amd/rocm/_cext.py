@@ -54,6 +54,7 @@ _SIGS = {
     "sqaod_rocm_dg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
     "sqaod_rocm_dg_annealer_get_counters": (_i32, [_vp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dg_annealer_get_form_energy_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
     "sqaod_rocm_dg_annealer_get_fields": (_i32, [_vp, _vp, _i32, _i32]),
     "sqaod_rocm_dg_annealer_get_phase_times": (_i32, [_vp, ctypes.POINTER(_f64), ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_bf_searcher_new": (_i32, [_i32, _PP]),
@@ -93,7 +94,22 @@ _SIGS = {
     "sqaod_rocm_bg_annealer_make_solution": (_i32, [_vp]),
     "sqaod_rocm_bg_annealer_get_system_E": (_i32, [_vp, _f64, _f64, ctypes.POINTER(_f64)]),
     "sqaod_rocm_bg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
+    "sqaod_rocm_formulas_new": (_i32, [_PP]),
+    "sqaod_rocm_formulas_delete": (_i32, [_vp]),
+    "sqaod_rocm_formulas_assign_device": (_i32, [_vp, _vp]),
+    "sqaod_rocm_dense_graph_batch_calculate_E": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _i32]),
+    "sqaod_rocm_dense_graph_calculate_hamiltonian": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i32]),
+    "sqaod_rocm_dense_graph_batch_calculate_E_from_spin": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _i32]),
+    "sqaod_rocm_bipartite_graph_batch_calculate_E": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp,
+                                                            _i32]),
+    "sqaod_rocm_bipartite_graph_batch_calculate_E_2d": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp, _i32,
+                                                               _vp, _i32]),
+    "sqaod_rocm_bipartite_graph_calculate_hamiltonian": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32,
+                                                                _vp]),
+    "sqaod_rocm_bipartite_graph_batch_calculate_E_from_spin": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp,
+                                                                      _vp, _vp, _i32]),
     "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
+    "sqaod_rocm_fp8_decode_table": (_i32, [_vp]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -162,6 +178,13 @@ def expf_checksum(lo=0, hi=1 << 32):
     out = _u64()
     _check(_lib.sqaod_rocm_expf_checksum(lo, hi, ctypes.byref(out)))
     return out.value
+
+
+def fp8_decode_table():
+    """The fp8 sweep's device decode of the e4m3 byte codes 0..255 (float32[256])."""
+    out = np.empty(256, np.float32)
+    _check(_lib.sqaod_rocm_fp8_decode_table(out.ctypes.data_as(ctypes.c_void_p)))
+    return out
 
 
 class _Device:
@@ -324,6 +347,12 @@ class _DenseGraphAnnealer:
     def get_energy_rounding_error(cobj, dtype):
         e = _f64()
         _check(_lib.sqaod_rocm_dg_annealer_get_energy_rounding_error(cobj, ctypes.byref(e)))
+        return e.value
+
+    @staticmethod
+    def get_form_energy_error(cobj, dtype):
+        e = _f64()
+        _check(_lib.sqaod_rocm_dg_annealer_get_form_energy_error(cobj, ctypes.byref(e)))
         return e.value
 
 
@@ -534,7 +563,92 @@ class _BipartiteGraphAnnealer:
         _check(_lib.sqaod_rocm_bg_annealer_anneal_one_step(cobj, float(G), float(beta)))
 
 
+class _Formulas:
+    """rocm_formulas module: formulas.inc:586-606 method table.  Arrays are
+    taken as given (the device-independent front end fixes types and shapes)."""
+
+    @staticmethod
+    def dg_formulas_new():
+        out = _vp()
+        _check(_lib.sqaod_rocm_formulas_new(ctypes.byref(out)))
+        return out.value
+
+    bg_formulas_new = dg_formulas_new
+
+    @staticmethod
+    def dg_formulas_delete(cobj):
+        _check(_lib.sqaod_rocm_formulas_delete(cobj))
+
+    bg_formulas_delete = dg_formulas_delete
+
+    @staticmethod
+    def dg_formulas_assign_device(cobj, devobj):
+        _check(_lib.sqaod_rocm_formulas_assign_device(cobj, devobj))
+
+    bg_formulas_assign_device = dg_formulas_assign_device
+
+    @staticmethod
+    def dense_graph_batch_calculate_E(cobj, E, W, x, dtype):
+        _check(_lib.sqaod_rocm_dense_graph_batch_calculate_E(cobj, _dt(dtype), _ptr(E), _ptr(W), W.shape[0], _ptr(x),
+                                                             x.shape[0]))
+
+    @staticmethod
+    def dense_graph_calculate_E(cobj, E, W, x, dtype):
+        _Formulas.dense_graph_batch_calculate_E(cobj, E, W, x.reshape(1, -1), dtype)
+
+    @staticmethod
+    def dense_graph_calculate_hamiltonian(cobj, h, J, c, W, dtype):
+        _check(_lib.sqaod_rocm_dense_graph_calculate_hamiltonian(cobj, _dt(dtype), _ptr(h), _ptr(J), _ptr(c), _ptr(W),
+                                                                 W.shape[0]))
+
+    @staticmethod
+    def dense_graph_batch_calculate_E_from_spin(cobj, E, h, J, c, q, dtype):
+        cc = np.array([c], dtype)
+        _check(_lib.sqaod_rocm_dense_graph_batch_calculate_E_from_spin(cobj, _dt(dtype), _ptr(E), _ptr(h), _ptr(J),
+                                                                       _ptr(cc), _ptr(q), J.shape[0], q.shape[0]))
+
+    @staticmethod
+    def dense_graph_calculate_E_from_spin(cobj, E, h, J, c, q, dtype):
+        _Formulas.dense_graph_batch_calculate_E_from_spin(cobj, E, h, J, c, q.reshape(1, -1), dtype)
+
+    @staticmethod
+    def bipartite_graph_batch_calculate_E(cobj, E, b0, b1, W, x0, x1, dtype):
+        _check(_lib.sqaod_rocm_bipartite_graph_batch_calculate_E(cobj, _dt(dtype), _ptr(E), _ptr(b0), b0.shape[0],
+                                                                 _ptr(b1), b1.shape[0], _ptr(W), _ptr(x0), _ptr(x1),
+                                                                 x0.shape[0]))
+
+    @staticmethod
+    def bipartite_graph_calculate_E(cobj, E, b0, b1, W, x0, x1, dtype):
+        _Formulas.bipartite_graph_batch_calculate_E(cobj, E, b0, b1, W, x0.reshape(1, -1), x1.reshape(1, -1), dtype)
+
+    @staticmethod
+    def bipartite_graph_batch_calculate_E_2d(cobj, E, b0, b1, W, x0, x1, dtype):
+        _check(_lib.sqaod_rocm_bipartite_graph_batch_calculate_E_2d(cobj, _dt(dtype), _ptr(E), _ptr(b0), b0.shape[0],
+                                                                    _ptr(b1), b1.shape[0], _ptr(W), _ptr(x0),
+                                                                    x0.shape[0], _ptr(x1), x1.shape[0]))
+
+    @staticmethod
+    def bipartite_graph_calculate_hamiltonian(cobj, h0, h1, J, c, b0, b1, W, dtype):
+        _check(_lib.sqaod_rocm_bipartite_graph_calculate_hamiltonian(cobj, _dt(dtype), _ptr(h0), _ptr(h1), _ptr(J),
+                                                                     _ptr(c), _ptr(b0), b0.shape[0], _ptr(b1),
+                                                                     b1.shape[0], _ptr(W)))
+
+    @staticmethod
+    def bipartite_graph_batch_calculate_E_from_spin(cobj, E, h0, h1, J, c, q0, q1, dtype):
+        cc = np.array([c], dtype)
+        _check(_lib.sqaod_rocm_bipartite_graph_batch_calculate_E_from_spin(cobj, _dt(dtype), _ptr(E), _ptr(h0),
+                                                                           h0.shape[0], _ptr(h1), h1.shape[0],
+                                                                           _ptr(J), _ptr(cc), _ptr(q0), _ptr(q1),
+                                                                           q0.shape[0]))
+
+    @staticmethod
+    def bipartite_graph_calculate_E_from_spin(cobj, E, h0, h1, J, c, q0, q1, dtype):
+        _Formulas.bipartite_graph_batch_calculate_E_from_spin(cobj, E, h0, h1, J, c, q0.reshape(1, -1),
+                                                              q1.reshape(1, -1), dtype)
+
+
 rocm_device = _Device
 rocm_dg_annealer = _DenseGraphAnnealer
 rocm_dg_bf_searcher = _DenseGraphBFSearcher
 rocm_bg_annealer = _BipartiteGraphAnnealer
+rocm_formulas = _Formulas

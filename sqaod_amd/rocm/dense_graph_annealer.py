@@ -34,6 +34,11 @@ class DenseGraphAnnealer(DenseGraphAnnealerBase):
         """max |E(dtype) - E(fp64)| of the last energy evaluation."""
         return cext.get_energy_rounding_error(self._cobj, self.dtype)
 
+    def get_form_energy_error(self):
+        """max over slices of |E_form(q) - E(q)| in FP64, E_form with the J the
+        field precision form anneals (bf16 hi + lo, fp8 J'), for the current q."""
+        return cext.get_form_energy_error(self._cobj, self.dtype)
+
 
 def dense_graph_annealer(W=None, optimize=minimize, dtype=np.float64, **prefs):
     """Factory of sqaod_amd.rocm.DenseGraphAnnealer (same signature as

@@ -1,5 +1,5 @@
 """Local fields of the dense annealer: the field GEMM F = Q.J on MFMA
-(fp32 and bf16 hi+lo) against the vector kernels and an exact/FP64 host
+(fp32, bf16 hi+lo and fp8) against the vector kernels and an exact/FP64 host
 product, and the sweep's incremental row updates against a recompute."""
 import numpy as np
 import pytest
@@ -28,12 +28,13 @@ def host_fields(oracle, W, q):
     return q.astype(np.float64) @ J.T
 
 
-@pytest.mark.parametrize("form", ["fp32", "bf16x2"])
-@pytest.mark.parametrize("N,m", [(300, 37), (128, 32), (1000, 16), (200, 520)])
+@pytest.mark.parametrize("form", ["fp32", "bf16x2", "fp8"])
+@pytest.mark.parametrize("N,m", [(300, 37), (128, 32), (1000, 16), (200, 520), (1100, 5)])
 def test_field_gemm_exact_on_integer_problems(oracle, rocm, form, N, m):
-    """Quarter-integer J: every engine gives the exact product (MFMA fp32,
-    MFMA bf16 hi+lo, vector) -- including ragged N and m, and m > 256 (the
-    MFMA kernels then take several passes over the slice rows)."""
+    """Quarter-integer J (|J| <= 2, exact in every form): every engine gives
+    the exact product (MFMA fp32, MFMA bf16 hi+lo, MFMA fp8, vector) --
+    including ragged N and m, m > 256 (the MFMA kernels then take several
+    passes over the slice rows) and an fp8 row stride (2048) above Np (1280)."""
     W = dense_int_W(N, SEED_BASE + N)
     res = {}
     for engine in ("mfma", "vector"):
@@ -61,7 +62,7 @@ def test_field_gemm_precision_forms(oracle, rocm):
     assert errs["bf16x2"] < 2e-5, errs
 
 
-@pytest.mark.parametrize("form", ["fp32", "bf16x2"])
+@pytest.mark.parametrize("form", ["fp32", "bf16x2", "fp8"])
 def test_incremental_fields_track_spins(oracle, rocm, form):
     """With refresh_interval=0 the sweep's row updates alone keep F = J.q
     exact for an integer problem over 40 annealing steps."""
