@@ -222,6 +222,36 @@ def run_serve(rocm, n_jobs, load, seed=20240601):
                     "arrivals of the stated rate"}
 
 
+def run_batch(rocm, n_inst, steps, seed=4242):
+    """Batching: n_inst dense instances, N = round(64 * 32^u) (log-uniform in
+    [64, 2048]), m uniform in {16, 32, 64, 128}, W symmetric i.i.d. U[-1, 1],
+    bf16x2 fields, the schedule's first `steps` steps captured as one hipGraph
+    after the autotuner has picked the launch shape."""
+    rng = np.random.default_rng(seed)
+    b = rocm.dense_batch(field_precision="bf16x2")
+    shapes = []
+    for i in range(n_inst):
+        N = int(round(64 * 32 ** rng.random()))
+        m = int(rng.choice([16, 32, 64, 128]))
+        U = rng.uniform(-1.0, 1.0, (N, N)).astype(np.float32)
+        W = np.triu(U) + np.triu(U, 1).T
+        b.add_qubo(W, m, 1000 + i)
+        shapes.append((N, m))
+    t0 = time.perf_counter()
+    b.prepare()
+    t_prep = time.perf_counter() - t0
+    G = np.array([schedule_G(k) for k in range(steps)])
+    wpb, largest = b.autotune(G[0], BETA, 2)
+    t0 = time.perf_counter()
+    ms = b.anneal(G, BETA, graph=True)
+    wall = time.perf_counter() - t0
+    att = b.attempts_per_step() * steps
+    return {"instances": n_inst, "steps": steps, "attempts_per_step": b.attempts_per_step(),
+            "attempts_per_s": att / (ms * 1e-3), "ms_per_step": ms / steps, "wall_s_incl_capture": wall,
+            "prepare_s": t_prep, "waves_per_block": wpb, "order": "largest_first" if largest else "given",
+            "sum_N_m": int(sum(N * m for N, m in shapes)), "max_N": max(N for N, _ in shapes)}
+
+
 def cpu_baseline(W, m, seconds, seed):
     """sqaod.cpu OpenMP coloring annealer (C restatement) on the host cores."""
     from oracle import oracle as O
@@ -284,6 +314,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the bipartite, BF search and serving runs")
     ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--batch-instances", type=int, default=512)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -350,6 +381,9 @@ def main():
         out["bipartite"] = {f: run_bipartite(rocm, args.steps, f) for f in ("fp32", "bf16x2")}
         print(f"# bipartite: {out['bipartite']['fp32']['attempts_per_s']:.4g} attempts/s", file=sys.stderr,
               flush=True)
+        out["batch"] = run_batch(rocm, args.batch_instances, 200)
+        print(f"# batch: {out['batch']['attempts_per_s']:.4g} attempts/s over {args.batch_instances} instances",
+              file=sys.stderr, flush=True)
         out["bf_search"] = [run_bf(rocm, n) for n in (24, 30)]
         print(f"# bf search N=30: {out['bf_search'][1]['candidates_per_s']:.4g} candidates/s", file=sys.stderr,
               flush=True)

@@ -35,6 +35,7 @@ typedef struct sqaod_rocm_dg_annealer sqaod_rocm_dg_annealer;
 typedef struct sqaod_rocm_dg_bf_searcher sqaod_rocm_dg_bf_searcher;
 typedef struct sqaod_rocm_bg_annealer sqaod_rocm_bg_annealer;
 typedef struct sqaod_rocm_formulas sqaod_rocm_formulas;
+typedef struct sqaod_rocm_dense_batch sqaod_rocm_dense_batch;
 
 /* ---- library ---- */
 
@@ -251,6 +252,41 @@ int sqaod_rocm_bipartite_graph_batch_calculate_E_from_spin(sqaod_rocm_formulas *
  * [lo, hi), for comparison with the host C library's expf:
  * sum of splitmix64((b << 32) | bits(expf(b))), NaN canonicalised. */
 int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsigned long long *out);
+
+/* ---------------- dense batch (no counterpart in the reference) ----------------
+ * For callers that anneal many independent dense QUBOs at once (hundreds of
+ * instances of uneven N and m): every instance is a float32 dense annealer
+ * (Philox streams, local-field sweep, MFMA refresh), and a step of the batch
+ * runs all of them in shared launches -- one grouped field GEMM, one grouped
+ * stream generation and one persistent sweep over every (instance, slice) --
+ * optionally captured as one hipGraph for a whole schedule.  Each instance's
+ * spins and energies are identical to a sqaod_rocm_dg_annealer run alone with
+ * the same seed, preferences and schedule. */
+int sqaod_rocm_dense_batch_new(sqaod_rocm_dense_batch **out);
+int sqaod_rocm_dense_batch_delete(sqaod_rocm_dense_batch *b);
+/* field_precision | refresh_interval | spin_limit | algorithm (applied to every
+ * instance), waves_per_block (0 = tuned, 1, 2, 4), order (largest_first | given) */
+int sqaod_rocm_dense_batch_set_preference(sqaod_rocm_dense_batch *b, const char *name, const char *value);
+/* W: N x N float32 symmetric QUBO; maximize != 0 maximizes; *index: the instance */
+int sqaod_rocm_dense_batch_add_qubo(sqaod_rocm_dense_batch *b, const float *W, int N, int m, unsigned long long seed,
+                                    int maximize, int *index);
+int sqaod_rocm_dense_batch_size(sqaod_rocm_dense_batch *b, int *n);
+/* prepare + randomize_spin of every instance */
+int sqaod_rocm_dense_batch_prepare(sqaod_rocm_dense_batch *b);
+/* nSteps steps with (G[k], beta[k]); graph != 0: one hipGraph for the whole
+ * schedule; *ms (may be null): device time of the schedule */
+int sqaod_rocm_dense_batch_anneal(sqaod_rocm_dense_batch *b, const double *G, const double *beta, int nSteps,
+                                  int graph, double *ms);
+/* times trialSteps steps at (G, beta) per candidate launch shape and keeps the
+ * fastest; spins, fields and streams are restored afterwards */
+int sqaod_rocm_dense_batch_autotune(sqaod_rocm_dense_batch *b, double G, double beta, int trialSteps,
+                                    int *wavesPerBlock, int *largestFirst);
+int sqaod_rocm_dense_batch_attempts_per_step(sqaod_rocm_dense_batch *b, long long *n);
+/* instance i: its shape, FP64-recomputed energies (m), solutions x (m x N) and spins q (m x N) */
+int sqaod_rocm_dense_batch_get_shape(sqaod_rocm_dense_batch *b, int i, int *N, int *m);
+int sqaod_rocm_dense_batch_get_E(sqaod_rocm_dense_batch *b, int i, float *E, int m);
+int sqaod_rocm_dense_batch_get_x(sqaod_rocm_dense_batch *b, int i, int8_t *x, int m, int N);
+int sqaod_rocm_dense_batch_get_q(sqaod_rocm_dense_batch *b, int i, int8_t *q, int m, int N);
 
 /* Self check without a reference counterpart: the fp8 sweep's e4m3 decode
  * (gfx950 v_cvt_pk_f32_fp8) applied to the byte codes 0..255 (out[256]). */

@@ -412,6 +412,100 @@ int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsig
     });
 }
 
+/* ---------------- dense batch ---------------- */
+
+namespace {
+DenseBatch &batchOf(sqaod_rocm_dense_batch *b) {
+    SQ_THROW_IF(b == nullptr, "null batch handle");
+    return *reinterpret_cast<DenseBatch *>(b);
+}
+DenseGraphAnnealer<float> &batchInstance(sqaod_rocm_dense_batch *b, int i, int m, int N) {
+    auto &a = batchOf(b).instance(i);
+    SQ_THROW_IF(m != a.numTrotters() || (N >= 0 && N != a.getProblemSize()), "buffer shape (%d, %d) != (%d, %d)", m,
+                N, a.numTrotters(), a.getProblemSize());
+    return a;
+}
+}  // namespace
+
+int sqaod_rocm_dense_batch_new(sqaod_rocm_dense_batch **out) {
+    return guarded([&] {
+        SQ_THROW_IF(!out, "null output");
+        *out = reinterpret_cast<sqaod_rocm_dense_batch *>(new DenseBatch(defaultDevice()));
+    });
+}
+
+int sqaod_rocm_dense_batch_delete(sqaod_rocm_dense_batch *b) {
+    return guarded([&] { delete &batchOf(b); });
+}
+
+int sqaod_rocm_dense_batch_set_preference(sqaod_rocm_dense_batch *b, const char *name, const char *value) {
+    return guarded([&] {
+        SQ_THROW_IF(!name || !value, "null preference");
+        batchOf(b).setPreference(name, value);
+    });
+}
+
+int sqaod_rocm_dense_batch_add_qubo(sqaod_rocm_dense_batch *b, const float *W, int N, int m, unsigned long long seed,
+                                    int maximize, int *index) {
+    return guarded([&] {
+        SQ_THROW_IF(!W || !index, "null argument");
+        *index = batchOf(b).addQUBO(W, N, m, seed, maximize ? optMaximize : optMinimize);
+    });
+}
+
+int sqaod_rocm_dense_batch_size(sqaod_rocm_dense_batch *b, int *n) {
+    return guarded([&] { *n = batchOf(b).size(); });
+}
+
+int sqaod_rocm_dense_batch_prepare(sqaod_rocm_dense_batch *b) {
+    return guarded([&] { batchOf(b).prepare(); });
+}
+
+int sqaod_rocm_dense_batch_anneal(sqaod_rocm_dense_batch *b, const double *G, const double *beta, int nSteps,
+                                  int graph, double *ms) {
+    return guarded([&] {
+        SQ_THROW_IF(!G || !beta, "null schedule");
+        batchOf(b).anneal(G, beta, nSteps, graph != 0);
+        if (ms)
+            *ms = batchOf(b).lastAnnealMs();
+    });
+}
+
+int sqaod_rocm_dense_batch_autotune(sqaod_rocm_dense_batch *b, double G, double beta, int trialSteps,
+                                    int *wavesPerBlock, int *largestFirst) {
+    return guarded([&] {
+        batchOf(b).autotune(G, beta, trialSteps);
+        if (wavesPerBlock)
+            *wavesPerBlock = batchOf(b).wavesPerBlock();
+        if (largestFirst)
+            *largestFirst = batchOf(b).largestFirst() ? 1 : 0;
+    });
+}
+
+int sqaod_rocm_dense_batch_attempts_per_step(sqaod_rocm_dense_batch *b, long long *n) {
+    return guarded([&] { *n = batchOf(b).attemptsPerStep(); });
+}
+
+int sqaod_rocm_dense_batch_get_shape(sqaod_rocm_dense_batch *b, int i, int *N, int *m) {
+    return guarded([&] {
+        auto &a = batchOf(b).instance(i);
+        *N = a.getProblemSize();
+        *m = a.numTrotters();
+    });
+}
+
+int sqaod_rocm_dense_batch_get_E(sqaod_rocm_dense_batch *b, int i, float *E, int m) {
+    return guarded([&] { batchInstance(b, i, m, -1).get_E(E); });
+}
+
+int sqaod_rocm_dense_batch_get_x(sqaod_rocm_dense_batch *b, int i, int8_t *x, int m, int N) {
+    return guarded([&] { batchInstance(b, i, m, N).get_x(x); });
+}
+
+int sqaod_rocm_dense_batch_get_q(sqaod_rocm_dense_batch *b, int i, int8_t *q, int m, int N) {
+    return guarded([&] { batchInstance(b, i, m, N).get_q(q); });
+}
+
 int sqaod_rocm_fp8_decode_table(float *out) {
     return guarded([&] {
         SQ_THROW_IF(!out, "null output");

@@ -598,24 +598,8 @@ void DenseGraphAnnealer<real>::refreshFields() {
     /* F = Q . J in the field format: MFMA for float solvers, vector FMA for
      * double (and on request, as the cross-check of the MFMA kernels) */
     const bool mfma = sizeof(real) == 4 && !refreshVector_;
-    if (mfma && !fragValid_) {
-        /* J in MFMA fragment order, built once per problem and form */
-        const float *J = (const float *)d_J_.ptr;
-        d_frag16_.release();
-        d_frag32_.release();
-        d_frag8_.release();
-        if (fieldFormat_ == fieldBF16x2) {
-            d_frag16_.allocate(fieldFragElemsBF16x2(Np_, Np_));
-            buildFragBF16x2(J, N_, Np_, Np_, Np_, d_frag16_.ptr, d_frag16_.n, dev_->stream());
-        } else if (fieldFormat_ == fieldFP8) {
-            d_frag8_.allocate(fieldFragElemsFP8(Np_, Np_));
-            buildFragFP8(d_J8_.ptr, N_, ld8_, Np_, Np_, d_frag8_.ptr, d_frag8_.n, dev_->stream());
-        } else {
-            d_frag32_.allocate(fieldFragElemsF32(Np_, Np_));
-            buildFragF32(J, N_, Np_, Np_, Np_, d_frag32_.ptr, d_frag32_.n, dev_->stream());
-        }
-        fragValid_ = true;
-    }
+    if (mfma)
+        ensureFragments();
     if (fieldFormat_ == fieldFP8) {
         /* the vector cross-check runs on J' as floats */
         if (mfma)
@@ -654,27 +638,24 @@ void DenseGraphAnnealer<real>::generateAttempts() {
 }
 
 template <class real>
-void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
-    /* CPUDenseGraphAnnealer.cpp:281-301 (coloring) and :354-371 (SA) */
-    throwErrorIfQNotSet();
-    ensureDevice();
-    checkAbort(false);
+bool DenseGraphAnnealer<real>::refreshDue() {
     if (!fieldsDirty_ && refreshInterval_ > 0 && stepsSinceRefresh_ >= refreshInterval_)
         fieldsDirty_ = true;
-    PhaseEvents pe;
-    if (fieldsDirty_) {
-        if (profile_ >= 2)
-            phaseBegin(0, pe);
-        refreshFields();
-        if (profile_ >= 2)
-            phaseEnd(pe);
-    }
-    if (profile_ >= 2)
-        phaseBegin(1, pe);
-    generateAttempts();
-    if (profile_ >= 2)
-        phaseEnd(pe);
+    return fieldsDirty_ && !plainSweep_;
+}
 
+template <class real>
+void DenseGraphAnnealer<real>::reserveTags(int n) {
+    /* decision words carry the step's tag; before the tags wrap, the words of
+     * every past step are cleared so none can match a new tag */
+    if ((unsigned long long)tag_ + (unsigned long long)n >= 0x7fffffffull) {
+        tag_ = 0;
+        HIP_CHECK(hipMemsetAsync(d_D_.ptr, 0, d_D_.bytes(), dev_->stream()));
+    }
+}
+
+template <class real>
+SweepArgs<real> DenseGraphAnnealer<real>::makeSweepArgs(real G, real beta) {
     SweepArgs<real> a{};
     a.N = N_;
     a.Np = Np_;
@@ -696,17 +677,43 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     a.X = d_X_.ptr;
     a.U = d_U_.ptr;
     a.D = d_D_.ptr;
+    reserveTags(1);
     a.tag = ++tag_;
-    if (tag_ >= 0x7fffffffu) {
-        tag_ = 1;
-        a.tag = 1;
-        HIP_CHECK(hipMemsetAsync(d_D_.ptr, 0, d_D_.bytes(), dev_->stream()));
-    }
     a.abortFlag = d_abort_.ptr;
     a.abortHost = d_abortHost_;
     a.counters = d_counters_.ptr;
     a.spinLimit = spinLimit_;
+    return a;
+}
 
+template <class real>
+void DenseGraphAnnealer<real>::stepDone() {
+    abortPending_ = true; /* the sweep reports an abort in h_abort_ itself */
+    ++stepsSinceRefresh_;
+    clearState(solSolutionAvailable);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
+    /* CPUDenseGraphAnnealer.cpp:281-301 (coloring) and :354-371 (SA) */
+    throwErrorIfQNotSet();
+    ensureDevice();
+    checkAbort(false);
+    PhaseEvents pe;
+    if (refreshDue()) {
+        if (profile_ >= 2)
+            phaseBegin(0, pe);
+        refreshFields();
+        if (profile_ >= 2)
+            phaseEnd(pe);
+    }
+    if (profile_ >= 2)
+        phaseBegin(1, pe);
+    generateAttempts();
+    if (profile_ >= 2)
+        phaseEnd(pe);
+
+    SweepArgs<real> a = makeSweepArgs(G, beta);
     int S = slicesPerBlock_;
     if (S <= 0) {
         S = 4;
@@ -725,10 +732,78 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
         phaseEnd(pe);
     if (pending_.size() > 3000)
         collectPhases();
+    stepDone();
+}
 
-    abortPending_ = true; /* the sweep reports an abort in h_abort_ itself */
-    ++stepsSinceRefresh_;
-    clearState(solSolutionAvailable);
+template <class real>
+bool DenseGraphAnnealer<real>::batchable(std::string *why) const {
+    const char *r = nullptr;
+    if (sizeof(real) != 4)
+        r = "float32 solvers only";
+    else if (rngMT_)
+        r = "rng=philox only";
+    else if (plainSweep_)
+        r = "sweep=local only";
+    else if (refreshVector_)
+        r = "refresh_engine=mfma only";
+    if (r && why)
+        *why = r;
+    return r == nullptr;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::ensureFragments() {
+    if (fragValid_ || sizeof(real) != 4)
+        return;
+    /* J in MFMA fragment order, built once per problem and form */
+    const float *J = (const float *)d_J_.ptr;
+    d_frag16_.release();
+    d_frag32_.release();
+    d_frag8_.release();
+    if (fieldFormat_ == fieldBF16x2) {
+        d_frag16_.allocate(fieldFragElemsBF16x2(Np_, Np_));
+        buildFragBF16x2(J, N_, Np_, Np_, Np_, d_frag16_.ptr, d_frag16_.n, dev_->stream());
+    } else if (fieldFormat_ == fieldFP8) {
+        d_frag8_.allocate(fieldFragElemsFP8(Np_, Np_));
+        buildFragFP8(d_J8_.ptr, N_, ld8_, Np_, Np_, d_frag8_.ptr, d_frag8_.n, dev_->stream());
+    } else {
+        d_frag32_.allocate(fieldFragElemsF32(Np_, Np_));
+        buildFragF32(J, N_, Np_, Np_, Np_, d_frag32_.ptr, d_frag32_.n, dev_->stream());
+    }
+    fragValid_ = true;
+}
+
+template <class real>
+FieldGemmJob DenseGraphAnnealer<real>::takeFieldGemmJob() {
+    ensureFragments();
+    FieldGemmJob j{};
+    j.frag = fieldFormat_ == fieldBF16x2 ? (const void *)d_frag16_.ptr
+             : fieldFormat_ == fieldFP8  ? (const void *)d_frag8_.ptr
+                                         : (const void *)d_frag32_.ptr;
+    j.colScale = d_scale_.ptr;
+    j.Qbits = d_Q_.ptr;
+    j.F = (float *)d_F_.ptr;
+    j.nCols = Np_;
+    j.K = Np_;
+    j.m = m_;
+    j.Nw = Nw_;
+    j.ldF = Np_;
+    checkFieldGemmJob(j, fieldFormat_);
+    fieldsDirty_ = false;
+    stepsSinceRefresh_ = 0;
+    return j;
+}
+
+template <class real>
+AttemptStreamJob DenseGraphAnnealer<real>::takeAttemptStreamJob() {
+    AttemptStreamJob j{};
+    j.seed = seed_;
+    j.sweepIdx = sweepIdx_++;
+    j.N = N_;
+    j.m = m_;
+    j.X = d_X_.ptr;
+    j.U = d_U_.ptr;
+    return j;
 }
 
 template <class real>

@@ -88,19 +88,16 @@ __device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int
     __syncthreads();
 }
 
-/* frag: [column blocks][K/32 k-blocks][2 planes][64 lanes] x 16 B; block cb
- * writes columns [16 cb, 16 cb + 16) of F (row stride ldF) */
-__global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const uint4 *frag, const uint32_t *Qbits,
-                                                                          int K, int m, int Nw, int ldF, float *F) {
-    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
-    __shared__ uint4 lut[256];                        /* byte -> 8 spins as bf16 +-1 (4 KiB) */
-    for (int i = threadIdx.x; i < 256; i += kBlockThreads)
-        lut[i] = __builtin_bit_cast(uint4, spinsBF16((uint32_t)i));
-    __syncthreads();
+typedef float RedTile[kTiles * 4][kWave];
+
+/* frag: [column blocks][K/32 k-blocks][2 planes][64 lanes] x 16 B; the block
+ * writes columns [16 cb, 16 cb + 16) of F (row stride ldF).  lut: byte -> 8
+ * spins as bf16 +-1, filled by the caller. */
+__device__ __forceinline__ void refreshBF16x2Block(const uint4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
+                                                   int ldF, float *F, int cb, RedTile *red, const uint4 *lut) {
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
-    const int cb = blockIdx.x;
     const int nK = K / 32;      /* k-blocks of 32; K is a multiple of 256 */
     const int nS = nK / kDepth; /* super-blocks */
     const int sOff = (int)(((long long)cb * 3) % nS);
@@ -171,14 +168,26 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const 
     }
 }
 
+__device__ __forceinline__ void fillLutBF16(uint4 *lut) {
+    for (int i = threadIdx.x; i < 256; i += kBlockThreads)
+        lut[i] = __builtin_bit_cast(uint4, spinsBF16((uint32_t)i));
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const uint4 *frag, const uint32_t *Qbits,
+                                                                          int K, int m, int Nw, int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
+    __shared__ uint4 lut[256];                        /* 4 KiB */
+    fillLutBF16(lut);
+    refreshBF16x2Block(frag, Qbits, K, m, Nw, ldF, F, blockIdx.x, red, lut);
+}
+
 /* frag: [column blocks][K/16 k-blocks][64 lanes] x float4 */
-__global__ __launch_bounds__(kBlockThreads) void fieldRefreshF32Kernel(const float4 *frag, const uint32_t *Qbits,
-                                                                       int K, int m, int Nw, int ldF, float *F) {
-    __shared__ float red[kSplitK][kTiles * 4][kWave];
+__device__ __forceinline__ void refreshF32Block(const float4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
+                                                int ldF, float *F, int cb, RedTile *red) {
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
-    const int cb = blockIdx.x;
     const int nK = K / 16; /* k-blocks of 16 (four 16x16x4 steps) */
     const int nS = nK / kDepth;
     const int sOff = (int)(((long long)cb * 3) % nS);
@@ -248,6 +257,12 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshF32Kernel(const flo
     }
 }
 
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshF32Kernel(const float4 *frag, const uint32_t *Qbits,
+                                                                       int K, int m, int Nw, int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave];
+    refreshF32Block(frag, Qbits, K, m, Nw, ldF, F, blockIdx.x, red);
+}
+
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 constexpr int kDepth8 = 2; /* fp8: k-blocks of 128 per super-block (256 k = 8 spin words) */
 constexpr int kE4M3Unit = 127; /* E8M0 block scale 2^0 */
@@ -268,18 +283,12 @@ __device__ __forceinline__ uint2 spinsFP8(uint32_t bits) {
  * k = 256 s + 64 g + 32 d + e, e < 32: spin word 8 s + 2 g + d of the row,
  * bit e -- so each lane's A operand for a whole super-block is the 8-byte
  * pair of words (8 s + 2 g, 8 s + 2 g + 1) of its row, loaded directly. */
-__global__ __launch_bounds__(kBlockThreads) void fieldRefreshFP8Kernel(const uint4 *frag, const float *colScale,
-                                                                       const uint32_t *Qbits, int K, int m, int Nw,
-                                                                       int ldF, float *F) {
-    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
-    __shared__ uint2 lut[256];                        /* byte -> 8 spins as e4m3 +-1 (2 KiB) */
-    for (int i = threadIdx.x; i < 256; i += kBlockThreads)
-        lut[i] = spinsFP8((uint32_t)i);
-    __syncthreads();
+__device__ __forceinline__ void refreshFP8Block(const uint4 *frag, const float *colScale, const uint32_t *Qbits,
+                                                int K, int m, int Nw, int ldF, float *F, int cb, RedTile *red,
+                                                const uint2 *lut) {
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
-    const int cb = blockIdx.x;
     const int nK = K / 128;
     const int nS = nK / kDepth8;
     const int sOff = (int)(((long long)cb * 3) % nS);
@@ -347,6 +356,57 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshFP8Kernel(const uin
             for (int r = 0; r < 4; ++r)
                 red[w][4 * t + r][lane] = acc[t][r];
         reduceStore(red, y0, m, cb, ldF, F, colScale);
+    }
+}
+
+__device__ __forceinline__ void fillLutFP8(uint2 *lut) {
+    for (int i = threadIdx.x; i < 256; i += kBlockThreads)
+        lut[i] = spinsFP8((uint32_t)i);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshFP8Kernel(const uint4 *frag, const float *colScale,
+                                                                       const uint32_t *Qbits, int K, int m, int Nw,
+                                                                       int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
+    __shared__ uint2 lut[256];                        /* 2 KiB */
+    fillLutFP8(lut);
+    refreshFP8Block(frag, colScale, Qbits, K, m, Nw, ldF, F, blockIdx.x, red, lut);
+}
+
+/* Many GEMMs of one form in one launch: block b runs column block
+ * b - blockStart[j] of job j, blockStart[j] <= b < blockStart[j + 1]. */
+__device__ __forceinline__ int jobOfBlock(const int *blockStart, int nJobs) {
+    int lo = 0, hi = nJobs - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blockStart[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int FORM>
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshGroupedKernel(const FieldGemmJob *jobs,
+                                                                           const int *blockStart, int nJobs) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave];
+    __shared__ uint4 lut[256];
+    const int j = jobOfBlock(blockStart, nJobs);
+    const FieldGemmJob jb = jobs[j];
+    const int cb = (int)blockIdx.x - blockStart[j];
+    if constexpr (FORM == fieldBF16x2) {
+        fillLutBF16(lut);
+        refreshBF16x2Block(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
+                           red, lut);
+    } else if constexpr (FORM == fieldFP8) {
+        fillLutFP8(reinterpret_cast<uint2 *>(lut));
+        refreshFP8Block(reinterpret_cast<const uint4 *>(jb.frag), jb.colScale, jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
+                        jb.F, cb, red, reinterpret_cast<const uint2 *>(lut));
+    } else {
+        refreshF32Block(reinterpret_cast<const float4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb, red);
     }
 }
 
@@ -438,6 +498,24 @@ void buildFragFP8(const uint8_t *d_B8, int rows, int ld8, int nCols, int K, uint
     buildFragFP8Kernel<<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(d_B8, rows, ld8, nCols / 16, K,
                                                                               reinterpret_cast<uint4 *>(d_frag));
     HIP_CHECK(hipGetLastError());
+}
+
+void fieldRefreshGrouped(int fieldFormat, const FieldGemmJob *d_jobs, const int *d_blockStart, int nJobs,
+                         int totalBlocks, hipStream_t stream) {
+    SQ_THROW_IF(nJobs <= 0 || totalBlocks <= 0, "grouped field GEMM: no work");
+    if (fieldFormat == fieldBF16x2)
+        fieldRefreshGroupedKernel<fieldBF16x2><<<totalBlocks, kBlockThreads, 0, stream>>>(d_jobs, d_blockStart, nJobs);
+    else if (fieldFormat == fieldFP8)
+        fieldRefreshGroupedKernel<fieldFP8><<<totalBlocks, kBlockThreads, 0, stream>>>(d_jobs, d_blockStart, nJobs);
+    else
+        fieldRefreshGroupedKernel<fieldF32><<<totalBlocks, kBlockThreads, 0, stream>>>(d_jobs, d_blockStart, nJobs);
+    HIP_CHECK(hipGetLastError());
+}
+
+void checkFieldGemmJob(const FieldGemmJob &j, int fieldFormat) {
+    checkGemm(j.nCols, j.K, j.K, j.Nw);
+    SQ_THROW_IF(j.ldF < j.nCols || !j.frag || !j.Qbits || !j.F || (fieldFormat == fieldFP8 && !j.colScale),
+                "grouped field GEMM: bad job");
 }
 
 void fieldRefreshFP8(const uint8_t *d_frag, size_t fragElems, const float *d_colScale, const uint32_t *d_Qbits,

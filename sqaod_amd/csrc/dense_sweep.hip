@@ -153,52 +153,40 @@ __device__ __forceinline__ void fp8x4ToFloat(uint32_t w, float v[4]) {
     v[0] = lo.x; v[1] = lo.y; v[2] = hi.x; v[3] = hi.y;
 }
 
+/* Bytes of LDS one wave's slice needs (fields, row stage, pending table,
+ * spin bits and the two mirrors), for a row length Np. */
 template <class real, int MODE>
-__global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
-                                                                                int xcdRemap) {
+__host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8) {
+    constexpr bool PLAIN = MODE == kModePlain;
+    constexpr bool FP8 = MODE == kModeFP8;
+    constexpr bool kDefer = !PLAIN && sizeof(real) == 4;
+    const size_t stageBytes = kDefer ? (FP8 ? (size_t)ld8 : (size_t)Np * sizeof(real)) : 0;
+    const size_t fieldBytes = PLAIN ? 0 : (size_t)Np * sizeof(real) + stageBytes;
+    return fieldBytes + (size_t)pendingTableSize(Np) * 4 + (size_t)3 * (Np / 32) * 4;
+}
+
+/* One wave runs the whole sweep of slice y: N attempts, with its LDS region
+ * p (sweepWaveBytes), the exp table and h already in LDS. */
+template <class real, int MODE>
+__device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y, const uint64_t *expTab,
+                                           const real *hS, unsigned char *p) {
 #pragma clang fp contract(off)
     constexpr bool PLAIN = MODE == kModePlain;
     constexpr bool FP8 = MODE == kModeFP8;
     static_assert(!FP8 || sizeof(real) == 4, "fp8 rows are for float solvers");
-    extern __shared__ __align__(16) unsigned char lds[];
     const int N = a.N, Np = a.Np, m = a.m, Nw = a.Nw;
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
     const int tblN = pendingTableSize(Np);
     const int tblMask = tblN - 1;
-
-    /* XCD-aware placement: consecutive slices on the same XCD (blocks are
-     * dispatched round-robin over the 8 XCDs). */
-    int b = blockIdx.x;
-    if (xcdRemap) {
-        int per = gridDim.x / 8;
-        b = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    }
-    const int y = b * slicesPerBlock + w;
-
-    /* LDS carve-up: exp table, h, then per wave F, pending table, spin bits, mirrors */
-    uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
-    real *hS = reinterpret_cast<real *>(lds + kExpTabBytes);
-    unsigned char *p = lds + kExpTabBytes + (size_t)Np * sizeof(real);
     constexpr bool kDefer = !PLAIN && sizeof(real) == 4; /* deferred row updates, staged in LDS */
     const size_t stageBytes = kDefer ? (FP8 ? (size_t)a.ld8 : (size_t)Np * sizeof(real)) : 0;
     const size_t fieldBytes = PLAIN ? 0 : (size_t)Np * sizeof(real) + stageBytes;
-    const size_t perWave = fieldBytes + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
-    p += perWave * w;
     real *F = reinterpret_cast<real *>(p);
     real *stage = F + Np; /* kDefer: the J row of the last flip, as loaded */
     const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(stage); /* FP8: the e4m3 row */
     int *tbl = reinterpret_cast<int *>(p + fieldBytes);
     uint32_t *Qb = reinterpret_cast<uint32_t *>(tbl + tblN);
     uint32_t *Mir[2] = {Qb + Nw, Qb + 2 * Nw};
-
-    if (threadIdx.x < 32)
-        expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
-    for (int i = threadIdx.x; i < Np; i += blockDim.x)
-        hS[i] = a.h[i];
-    __syncthreads();
-    if (y >= m)
-        return;
 
     const long long yN = (long long)y * N;
     const int n0 = (y == 0) ? m - 1 : y - 1;
@@ -687,6 +675,78 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     }
 }
 
+template <class real, int MODE>
+__global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
+                                                                                int xcdRemap) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int w = threadIdx.x / kWave;
+
+    /* XCD-aware placement: consecutive slices on the same XCD (blocks are
+     * dispatched round-robin over the 8 XCDs). */
+    int b = blockIdx.x;
+    if (xcdRemap) {
+        int per = gridDim.x / 8;
+        b = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    }
+    const int y = b * slicesPerBlock + w;
+
+    /* LDS carve-up: exp table, h, then one region per wave */
+    uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
+    real *hS = reinterpret_cast<real *>(lds + kExpTabBytes);
+    unsigned char *p = lds + kExpTabBytes + (size_t)a.Np * sizeof(real) + sweepWaveBytes<real, MODE>(a.Np, a.ld8) * w;
+    if (threadIdx.x < 32)
+        expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
+    for (int i = threadIdx.x; i < a.Np; i += blockDim.x)
+        hS[i] = a.h[i];
+    __syncthreads();
+    if (y >= a.m)
+        return;
+    sweepSlice<real, MODE>(a, y, expTab, hS, p);
+}
+
+/* Many instances in one persistent launch: every wave takes (instance,
+ * slice) items from a queue in item order until it is empty, loads the
+ * instance's h into its own LDS region and runs the slice's sweep.  Items of
+ * one instance are consecutive, so the waves running them start together;
+ * a wave that waits on a neighbour slice waits only on items already taken
+ * (by running waves) or next in the queue.  With at least max(m) waves
+ * resident every instance therefore completes: the items taken are a prefix
+ * of the queue, every instance before the last started one has all its
+ * slices running, and the last one's remaining slices are taken as soon as
+ * any earlier instance frees its waves.  regionBytes is the per-wave region
+ * (h + sweepWaveBytes) for the largest instance. */
+template <class real, int MODE>
+__global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepBatchKernel(const SweepArgs<real> *args,
+                                                                                     const uint2 *items, int nItems,
+                                                                                     int *queue, int regionBytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
+    unsigned char *region = lds + kExpTabBytes + (size_t)regionBytes * w;
+    if (threadIdx.x < 32)
+        expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
+    __syncthreads();
+    for (;;) {
+        int item = 0;
+        if (lane == 0)
+            item = atomicAdd(queue, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0));
+        if (item >= nItems)
+            break;
+        const uint2 it = items[item];
+        const SweepArgs<real> a = args[__builtin_amdgcn_readfirstlane((int)it.x)];
+        real *hS = reinterpret_cast<real *>(region);
+        for (int i = lane; i < a.Np; i += kWave)
+            hS[i] = a.h[i];
+        waveSync();
+        sweepSlice<real, MODE>(a, (int)it.y, expTab, hS, region + (size_t)a.Np * sizeof(real));
+        waveSync();
+    }
+}
+
+__global__ void resetQueueKernel(int *queue) { *queue = 0; }
+
 __global__ void fp8DecodeKernel(float *out) {
     const int t = threadIdx.x; /* 64 threads, 4 codes each */
     float v[4];
@@ -710,14 +770,75 @@ void fp8DecodeTable(float out[256]) {
     HIP_CHECK(e);
 }
 
-size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, int ld8) {
-    int tblN = pendingTableSize(Np);
-    int Nw = Np / 32;
-    const bool defer = !plain && realSize == 4;
-    const size_t stage = defer ? (fieldFormat == fieldFP8 ? (size_t)ld8 : (size_t)Np * realSize) : 0;
-    size_t perWave = (plain ? 0 : (size_t)Np * realSize + stage) + (size_t)tblN * 4 + (size_t)3 * Nw * 4;
-    return kExpTabBytes + (size_t)Np * realSize + perWave * slicesPerBlock;
+namespace {
+size_t waveBytesFor(int Np, size_t realSize, bool plain, int fieldFormat, int ld8) {
+    if (realSize == 8)
+        return plain ? sweepWaveBytes<double, kModePlain>(Np, ld8) : sweepWaveBytes<double, kModeLocal>(Np, ld8);
+    if (plain)
+        return sweepWaveBytes<float, kModePlain>(Np, ld8);
+    return fieldFormat == fieldFP8 ? sweepWaveBytes<float, kModeFP8>(Np, ld8) : sweepWaveBytes<float, kModeLocal>(Np, ld8);
 }
+
+template <class real>
+void *batchKernelFor(bool plain, int fieldFormat) {
+    if (plain)
+        return (void *)denseSweepBatchKernel<real, kModePlain>;
+    if constexpr (sizeof(real) == 4)
+        if (fieldFormat == fieldFP8)
+            return (void *)denseSweepBatchKernel<real, kModeFP8>;
+    return (void *)denseSweepBatchKernel<real, kModeLocal>;
+}
+}  // namespace
+
+size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, int ld8) {
+    return kExpTabBytes + (size_t)Np * realSize + waveBytesFor(Np, realSize, plain, fieldFormat, ld8) * slicesPerBlock;
+}
+
+size_t denseSweepBatchRegionBytes(int maxNp, int maxLd8, size_t realSize, bool plain, int fieldFormat) {
+    return (size_t)maxNp * realSize + waveBytesFor(maxNp, realSize, plain, fieldFormat, maxLd8);
+}
+
+template <class real>
+int denseSweepBatchBlocks(size_t regionBytes, int wavesPerBlock, bool plain, int fieldFormat, int numCUs) {
+    SQ_THROW_IF(wavesPerBlock < 1 || wavesPerBlock > kMaxSlicesPerBlock, "waves per block out of range (1..%d)",
+                kMaxSlicesPerBlock);
+    const size_t lds = kExpTabBytes + regionBytes * wavesPerBlock;
+    void *fn = batchKernelFor<real>(plain, fieldFormat);
+    HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int perCU = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, wavesPerBlock * kWave, lds));
+    return perCU * numCUs;
+}
+
+template <class real>
+void denseSweepBatch(const SweepArgs<real> *d_args, const uint2 *d_items, int nItems, int *d_queue, bool plain,
+                     int fieldFormat, size_t regionBytes, int wavesPerBlock, int nBlocks, hipStream_t stream) {
+    SQ_THROW_IF(nItems <= 0 || nBlocks <= 0, "batch sweep: nothing to run");
+    SQ_THROW_IF(regionBytes % 16 != 0, "batch sweep: region of %zu bytes is not 16-byte aligned", regionBytes);
+    /* the LDS attribute was set by denseSweepBatchBlocks (nothing but stream
+     * work here, so the launch can be captured into a graph) */
+    const size_t lds = kExpTabBytes + regionBytes * wavesPerBlock;
+    resetQueueKernel<<<1, 1, 0, stream>>>(d_queue);
+    HIP_CHECK(hipGetLastError());
+    const int rb = (int)regionBytes;
+    if (plain)
+        denseSweepBatchKernel<real, kModePlain><<<nBlocks, wavesPerBlock * kWave, lds, stream>>>(d_args, d_items, nItems,
+                                                                                                d_queue, rb);
+    else if (sizeof(real) == 4 && fieldFormat == fieldFP8)
+        denseSweepBatchKernel<real, (sizeof(real) == 4 ? kModeFP8 : kModeLocal)>
+            <<<nBlocks, wavesPerBlock * kWave, lds, stream>>>(d_args, d_items, nItems, d_queue, rb);
+    else
+        denseSweepBatchKernel<real, kModeLocal><<<nBlocks, wavesPerBlock * kWave, lds, stream>>>(d_args, d_items, nItems,
+                                                                                                d_queue, rb);
+    HIP_CHECK(hipGetLastError());
+}
+
+template int denseSweepBatchBlocks<float>(size_t, int, bool, int, int);
+template int denseSweepBatchBlocks<double>(size_t, int, bool, int, int);
+template void denseSweepBatch<float>(const SweepArgs<float> *, const uint2 *, int, int *, bool, int, size_t, int, int,
+                                     hipStream_t);
+template void denseSweepBatch<double>(const SweepArgs<double> *, const uint2 *, int, int *, bool, int, size_t, int,
+                                      int, hipStream_t);
 
 template <class real>
 void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, hipStream_t stream) {

@@ -45,6 +45,20 @@ template <class real>
 void philoxAttempts(unsigned long long seed, unsigned long long sweepIdx, int N, int m, uint32_t *d_X,
                     real *d_U, hipStream_t stream);
 
+/* The same streams for many sweeps in one launch (values identical to
+ * philoxAttempts with the job's seed and sweep index): job j owns the
+ * kAttemptBlock-thread blocks [blockStart[j], blockStart[j] + ceil(N m / kAttemptBlock)). */
+constexpr int kAttemptBlock = 256;
+struct AttemptStreamJob {
+    unsigned long long seed, sweepIdx;
+    int N, m;
+    uint32_t *X;
+    void *U; /* real */
+};
+template <class real>
+void philoxAttemptsGrouped(const AttemptStreamJob *d_jobs, const int *d_blockStart, int nJobs, int totalBlocks,
+                           hipStream_t stream);
+
 /* randomizeSpin from a word stream: bit (y, x) = word[y*rowStride+x] & 1
  * (CPUDenseGraphAnnealer.cpp:152-161: randInt(2) ? +1 : -1; the bipartite
  * annealer interleaves the q0 and q1 rows of each slice,
@@ -113,6 +127,20 @@ void fieldRefreshF32MFMA(const float *d_frag, size_t fragElems, const uint32_t *
  * on v_mfma_scale_f32_16x16x128_f8f6f4 (unit block scales), fp32
  * accumulation.  fieldFragElemsFP8 = nCols K bytes. */
 size_t fieldFragElemsFP8(int nCols, int K);
+/* One GEMM of a grouped launch (fragments already built; colScale for fp8). */
+struct FieldGemmJob {
+    const void *frag;
+    const float *colScale;
+    const uint32_t *Qbits;
+    float *F;
+    int nCols, K, m, Nw, ldF;
+};
+/* All jobs (one field format) in one launch: job j owns blocks
+ * [blockStart[j], blockStart[j] + nCols_j / 16); totalBlocks is the sum.
+ * checkFieldGemmJob validates a job's shape on the host. */
+void checkFieldGemmJob(const FieldGemmJob &job, int fieldFormat);
+void fieldRefreshGrouped(int fieldFormat, const FieldGemmJob *d_jobs, const int *d_blockStart, int nJobs,
+                         int totalBlocks, hipStream_t stream);
 void buildFragFP8(const uint8_t *d_B8, int rows, int ld8, int nCols, int K, uint8_t *d_frag, size_t fragElems,
                   hipStream_t stream);
 void fieldRefreshFP8(const uint8_t *d_frag, size_t fragElems, const float *d_colScale, const uint32_t *d_Qbits,
@@ -209,6 +237,20 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
  * stride, used when fieldFormat == fieldFP8). */
 size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat = fieldF32,
                           int ld8 = 0);
+
+/* Many instances in one persistent launch (one wave per (instance, slice)
+ * item, taken from a queue in item order; items of one instance must be
+ * consecutive and every instance's m must not exceed the resident waves).
+ * d_args: one SweepArgs per instance (all of one field format), d_items:
+ * (instance, slice) pairs, d_queue: one int, reset by the launcher.
+ * regionBytes = denseSweepBatchRegionBytes for the largest instance. */
+size_t denseSweepBatchRegionBytes(int maxNp, int maxLd8, size_t realSize, bool plain, int fieldFormat);
+/* Blocks of wavesPerBlock waves that stay resident together on the device. */
+template <class real>
+int denseSweepBatchBlocks(size_t regionBytes, int wavesPerBlock, bool plain, int fieldFormat, int numCUs);
+template <class real>
+void denseSweepBatch(const SweepArgs<real> *d_args, const uint2 *d_items, int nItems, int *d_queue, bool plain,
+                     int fieldFormat, size_t regionBytes, int wavesPerBlock, int nBlocks, hipStream_t stream);
 
 /* The sweep's e4m3 decode applied to the 256 byte codes (a self check). */
 void fp8DecodeTable(float out[256]);

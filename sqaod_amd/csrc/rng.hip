@@ -118,17 +118,44 @@ __global__ void streamToAttemptsKernel(const uint32_t *words, int N, int m, int 
     U[idx] = Draws<real>::uniform(w);
 }
 
+/* attempt idx = y N + k of a sweep: site and uniform from Philox keyed
+ * (seed) at counter (k, y, sweep index) */
 template <class real>
-__global__ void philoxAttemptsKernel(uint32_t k0, uint32_t k1, uint32_t s0, uint32_t s1, int N, int m, uint32_t *X,
-                                     real *U) {
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)N * m;
-    if (idx >= total)
-        return;
+__device__ __forceinline__ void philoxAttempt(uint32_t k0, uint32_t k1, uint32_t s0, uint32_t s1, int N, long long idx,
+                                              uint32_t *X, real *U) {
     int y = (int)(idx / N), k = (int)(idx % N);
     U4 r = philox4x32_10(U4{(uint32_t)k, (uint32_t)y, s0, s1}, k0, k1);
     X[idx] = r.x % (uint32_t)N;
     U[idx] = Draws<real>::uniform(r);
+}
+
+template <class real>
+__global__ void philoxAttemptsKernel(uint32_t k0, uint32_t k1, uint32_t s0, uint32_t s1, int N, int m, uint32_t *X,
+                                     real *U) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)N * m)
+        return;
+    philoxAttempt<real>(k0, k1, s0, s1, N, idx, X, U);
+}
+
+/* many sweeps' streams in one launch: job j owns blocks [blockStart[j], ...) */
+template <class real>
+__global__ void philoxAttemptsGroupedKernel(const AttemptStreamJob *jobs, const int *blockStart, int nJobs) {
+    int lo = 0, hi = nJobs - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blockStart[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const AttemptStreamJob j = jobs[lo];
+    const long long idx = (long long)(b - blockStart[lo]) * blockDim.x + threadIdx.x;
+    if (idx >= (long long)j.N * j.m)
+        return;
+    philoxAttempt<real>((uint32_t)j.seed, (uint32_t)(j.seed >> 32), (uint32_t)j.sweepIdx, (uint32_t)(j.sweepIdx >> 32),
+                        j.N, idx, j.X, reinterpret_cast<real *>(j.U));
 }
 
 __global__ void spinsFromWordsKernel(const uint32_t *words, long long rowStride, int N, int m, int Nw, uint32_t *Q) {
@@ -210,6 +237,16 @@ void streamToAttempts(const uint32_t *d_words, int N, int m, int order, uint32_t
     streamToAttemptsKernel<real><<<gridFor(total, 256), 256, 0, stream>>>(d_words, N, m, order, d_X, d_U);
     HIP_CHECK(hipGetLastError());
 }
+
+template <class real>
+void philoxAttemptsGrouped(const AttemptStreamJob *d_jobs, const int *d_blockStart, int nJobs, int totalBlocks,
+                           hipStream_t stream) {
+    SQ_THROW_IF(nJobs <= 0 || totalBlocks <= 0, "grouped attempt streams: no work");
+    philoxAttemptsGroupedKernel<real><<<totalBlocks, kAttemptBlock, 0, stream>>>(d_jobs, d_blockStart, nJobs);
+    HIP_CHECK(hipGetLastError());
+}
+template void philoxAttemptsGrouped<float>(const AttemptStreamJob *, const int *, int, int, hipStream_t);
+template void philoxAttemptsGrouped<double>(const AttemptStreamJob *, const int *, int, int, hipStream_t);
 
 template <class real>
 void philoxAttempts(unsigned long long seed, unsigned long long sweepIdx, int N, int m, uint32_t *d_X, real *d_U,

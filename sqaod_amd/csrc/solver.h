@@ -149,6 +149,36 @@ public:
      * fp8 J'), E the J as given; 0 for the fp32/fp64 form. */
     double formEnergyError();
 
+    /* Hooks of the batch engine (dense_batch.cpp), which runs many
+     * instances' steps in shared launches with the same kernels and the same
+     * arguments annealOneStep uses. */
+    bool batchable(std::string *why) const; /* float, philox, local sweep, MFMA refresh */
+    bool refreshDue();                       /* the refresh policy of annealOneStep */
+    void reserveTags(int n);                 /* clears the decision words before the tag wraps */
+    FieldGemmJob takeFieldGemmJob();         /* the step's field refresh, enqueued by the caller */
+    AttemptStreamJob takeAttemptStreamJob(); /* the step's Philox streams (advances the sweep index) */
+    SweepArgs<real> makeSweepArgs(real G, real beta);
+    void stepDone();
+    void checkStepAbort() { checkAbort(false); }
+    int fieldFormat() const { return fieldFormat_; }
+    int paddedN() const { return Np_; }
+    int fp8RowStrideBytes() const { return ld8_; }
+    Device *device() const { return dev_; }
+    /* spins and fields, for snapshots around trial steps */
+    uint32_t *spinWords() { return d_Q_.ptr; }
+    real *fields() { return d_F_.ptr; }
+    struct StepState {
+        unsigned long long sweepIdx;
+        int stepsSinceRefresh;
+        bool fieldsDirty;
+    };
+    StepState stepState() const { return {sweepIdx_, stepsSinceRefresh_, fieldsDirty_}; }
+    void setStepState(const StepState &s) {
+        sweepIdx_ = s.sweepIdx;
+        stepsSinceRefresh_ = s.stepsSinceRefresh;
+        fieldsDirty_ = s.fieldsDirty;
+    }
+
 private:
     void ensureDevice();
     void checkAbort(bool synchronous);
@@ -157,6 +187,7 @@ private:
     void generateAttempts();
     void uploadJ(const real *J, const real *h, real c);
     void prepareFieldForm(); /* derives the form's J (bf16 split / fp8 quantization) from d_J_ */
+    void ensureFragments();  /* J in MFMA fragment order for the field form */
 
     Device *dev_ = nullptr;
     int N_ = 0, Np_ = 0, Nw_ = 0, m_ = -1;
@@ -219,6 +250,60 @@ private:
     std::vector<real> h_;     /* host h (for getHamiltonian) */
     std::vector<real> E_;
     std::vector<int8_t> q_;   /* m x N after makeSolution */
+};
+
+/* ------------------------------------------------------------------ */
+/* Batch of dense annealers stepped together (no reference counterpart) */
+/* ------------------------------------------------------------------ */
+/* Many independent dense instances (float, any N and m) advance through
+ * one schedule together: per step one grouped field-GEMM launch, one
+ * grouped Philox launch and one persistent sweep launch over every
+ * (instance, slice), optionally captured as one hipGraph for the whole
+ * schedule.  Each instance is a DenseGraphAnnealer<float> and its results
+ * are bit-identical to running it alone with the same seed and preferences. */
+class DenseBatch {
+public:
+    explicit DenseBatch(Device &dev);
+    ~DenseBatch();
+    /* batch-wide preferences: field_precision, refresh_interval, spin_limit,
+     * algorithm (as the annealer's), waves_per_block (1, 2, 4; 0 = tuned),
+     * order ("largest_first" | "given") */
+    void setPreference(const std::string &name, const std::string &value);
+    PrefList getPreferences() const;
+    /* setQUBO + n_trotters + seed on a new instance; returns its index */
+    int addQUBO(const float *W, int N, int m, unsigned long long seed, OptimizeMethod om);
+    int size() const { return (int)inst_.size(); }
+    DenseGraphAnnealer<float> &instance(int i);
+    void prepare(); /* prepare + randomizeSpin of every instance */
+    /* steps k = 0..nSteps-1 with (G[k], beta[k]) on every instance; graph:
+     * capture the whole schedule as one hipGraph and launch it once */
+    void anneal(const double *G, const double *beta, int nSteps, bool graph);
+    /* times trialSteps steps of each (waves per block, order) candidate at
+     * (G, beta) and keeps the fastest; the instances' spins, fields and
+     * streams are restored afterwards, so results do not depend on it */
+    void autotune(double G, double beta, int trialSteps);
+    long long attemptsPerStep() const;
+    int wavesPerBlock() const { return wavesPerBlock_; }
+    bool largestFirst() const { return largestFirst_; }
+    double lastAnnealMs() const { return lastMs_; }
+
+private:
+    void buildItems();
+    void run(const double *G, const double *beta, int nSteps, bool graph, int wpb);
+
+    Device *dev_;
+    std::vector<DenseGraphAnnealer<float> *> inst_;
+    std::vector<std::pair<std::string, std::string>> prefs_;
+    int fieldFormat_ = fieldF32;
+    int wavesPerBlock_ = 0; /* 0: not tuned yet (1 is used) */
+    bool largestFirst_ = true;
+    bool itemsValid_ = false;
+    double lastMs_ = 0.;
+    DevBuf<uint2> d_items_;
+    DevBuf<int> d_queue_;
+    DevBuf<unsigned char> d_plan_;
+    PinnedBuf<unsigned char> h_plan_;
+    int nItems_ = 0, maxNp_ = 0, maxLd8_ = 0, maxM_ = 0;
 };
 
 /* ------------------------------------------------------------------ */

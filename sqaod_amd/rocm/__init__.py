@@ -6,3 +6,4 @@ from .device import create_device, Device  # noqa: F401
 from .dense_graph_bf_searcher import dense_graph_bf_searcher, DenseGraphBFSearcher  # noqa: F401
 from .bipartite_graph_annealer import bipartite_graph_annealer, BipartiteGraphAnnealer  # noqa: F401
 from . import formulas  # noqa: F401
+from .dense_batch import dense_batch, DenseBatch  # noqa: F401

@@ -110,6 +110,19 @@ _SIGS = {
                                                                       _vp, _vp, _i32]),
     "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
     "sqaod_rocm_fp8_decode_table": (_i32, [_vp]),
+    "sqaod_rocm_dense_batch_new": (_i32, [_PP]),
+    "sqaod_rocm_dense_batch_delete": (_i32, [_vp]),
+    "sqaod_rocm_dense_batch_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "sqaod_rocm_dense_batch_add_qubo": (_i32, [_vp, _vp, _i32, _i32, _u64, _i32, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dense_batch_size": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dense_batch_prepare": (_i32, [_vp]),
+    "sqaod_rocm_dense_batch_anneal": (_i32, [_vp, _vp, _vp, _i32, _i32, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dense_batch_autotune": (_i32, [_vp, _f64, _f64, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dense_batch_attempts_per_step": (_i32, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "sqaod_rocm_dense_batch_get_shape": (_i32, [_vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaod_rocm_dense_batch_get_E": (_i32, [_vp, _i32, _vp, _i32]),
+    "sqaod_rocm_dense_batch_get_x": (_i32, [_vp, _i32, _vp, _i32, _i32]),
+    "sqaod_rocm_dense_batch_get_q": (_i32, [_vp, _i32, _vp, _i32, _i32]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -647,8 +660,95 @@ class _Formulas:
                                                               q1.reshape(1, -1), dtype)
 
 
+class _DenseBatch:
+    """ctypes front of sqaod_rocm_dense_batch_* (float32 instances)."""
+
+    @staticmethod
+    def new():
+        p = _vp()
+        _check(_lib.sqaod_rocm_dense_batch_new(ctypes.byref(p)))
+        return p
+
+    @staticmethod
+    def delete(cobj):
+        _check(_lib.sqaod_rocm_dense_batch_delete(cobj))
+
+    @staticmethod
+    def set_preferences(cobj, prefs):
+        for k, v in prefs.items():
+            _check(_lib.sqaod_rocm_dense_batch_set_preference(cobj, str(k).encode(), str(v).encode()))
+
+    @staticmethod
+    def add_qubo(cobj, W, m, seed, maximize):
+        W = np.ascontiguousarray(W, np.float32)
+        idx = _i32()
+        _check(_lib.sqaod_rocm_dense_batch_add_qubo(cobj, _ptr(W), W.shape[0], int(m), int(seed) & ((1 << 64) - 1),
+                                                    1 if maximize else 0, ctypes.byref(idx)))
+        return idx.value
+
+    @staticmethod
+    def size(cobj):
+        n = _i32()
+        _check(_lib.sqaod_rocm_dense_batch_size(cobj, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def prepare(cobj):
+        _check(_lib.sqaod_rocm_dense_batch_prepare(cobj))
+
+    @staticmethod
+    def anneal(cobj, G, beta, graph):
+        G = np.ascontiguousarray(G, np.float64)
+        beta = np.ascontiguousarray(beta, np.float64)
+        ms = _f64()
+        _check(_lib.sqaod_rocm_dense_batch_anneal(cobj, _ptr(G), _ptr(beta), G.shape[0], 1 if graph else 0,
+                                                  ctypes.byref(ms)))
+        return ms.value
+
+    @staticmethod
+    def autotune(cobj, G, beta, steps):
+        w, o = _i32(), _i32()
+        _check(_lib.sqaod_rocm_dense_batch_autotune(cobj, float(G), float(beta), int(steps), ctypes.byref(w),
+                                                    ctypes.byref(o)))
+        return w.value, bool(o.value)
+
+    @staticmethod
+    def attempts_per_step(cobj):
+        n = ctypes.c_longlong()
+        _check(_lib.sqaod_rocm_dense_batch_attempts_per_step(cobj, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def shape(cobj, i):
+        N, m = _i32(), _i32()
+        _check(_lib.sqaod_rocm_dense_batch_get_shape(cobj, i, ctypes.byref(N), ctypes.byref(m)))
+        return N.value, m.value
+
+    @staticmethod
+    def get_E(cobj, i):
+        N, m = _DenseBatch.shape(cobj, i)
+        E = np.empty(m, np.float32)
+        _check(_lib.sqaod_rocm_dense_batch_get_E(cobj, i, _ptr(E), m))
+        return E
+
+    @staticmethod
+    def get_x(cobj, i):
+        N, m = _DenseBatch.shape(cobj, i)
+        x = np.empty((m, N), np.int8)
+        _check(_lib.sqaod_rocm_dense_batch_get_x(cobj, i, _ptr(x), m, N))
+        return x
+
+    @staticmethod
+    def get_q(cobj, i):
+        N, m = _DenseBatch.shape(cobj, i)
+        q = np.empty((m, N), np.int8)
+        _check(_lib.sqaod_rocm_dense_batch_get_q(cobj, i, _ptr(q), m, N))
+        return q
+
+
 rocm_device = _Device
 rocm_dg_annealer = _DenseGraphAnnealer
 rocm_dg_bf_searcher = _DenseGraphBFSearcher
 rocm_bg_annealer = _BipartiteGraphAnnealer
 rocm_formulas = _Formulas
+rocm_dense_batch = _DenseBatch
