@@ -163,63 +163,97 @@ def run_bf(rocm, N, seed=1133557):
             "n_solutions": len(s.get_x())}
 
 
-def run_serve(rocm, n_jobs, load, seed=20240601):
-    """Serving: jobs of 70% dense / 30% bipartite (N0 = N1 = N/2) problems,
-    N log-uniform in [32, 8192], m uniform in {16..256}, 50..500 steps of the
-    schedule, weights N(0,1).  Each job runs end to end on the device's stream
-    (set_qubo, prepare, randomize_spin, the steps, get_E) and its service time
-    is measured.  Arrivals are Poisson at the rate giving `load` utilisation;
-    latency = completion - arrival of a FIFO single-stream server replayed
-    over the measured service times."""
+def serve_jobs(n_jobs, seed):
+    """70% dense / 30% bipartite (N0 = N1 = N/2) jobs, N log-uniform in
+    [32, 8192], m uniform in {16..256}, weights N(0,1), each with its own
+    schedule: 50..500 steps of G from G0 ~ U[2, 5] down to 0.01 geometrically,
+    beta ~ U[20, 60]."""
     rng = np.random.default_rng(seed)
     jobs = []
-    for _ in range(n_jobs):
+    for i in range(n_jobs):
         N = int(round(np.exp(rng.uniform(np.log(32), np.log(8192)))))
         m = int(rng.integers(16, 257))
         steps = int(rng.integers(50, 501))
+        G0, beta = float(rng.uniform(2.0, 5.0)), float(rng.uniform(20.0, 60.0))
+        G = G0 * (0.01 / G0) ** (np.arange(steps) / max(steps - 1, 1))
         if rng.random() < 0.7:
             U = rng.standard_normal((N, N)).astype(np.float32)
             W = np.triu(U) + np.triu(U, 1).T
-            jobs.append(("dense", N, m, steps, (W,)))
+            jobs.append(("dense", (W,), m, 5000 + i, G, beta, N * m * steps))
         else:
             h = max(N // 2, 1)
-            jobs.append(("bipartite", N, m, steps, (rng.standard_normal(h).astype(np.float32),
-                                                    rng.standard_normal(h).astype(np.float32),
-                                                    rng.standard_normal((h, h)).astype(np.float32))))
-    dev = rocm.device.get_active_device()
-    service, attempts = [], 0
-    for kind, N, m, steps, data in jobs:
-        t0 = time.perf_counter()
+            jobs.append(("bipartite", (rng.standard_normal(h).astype(np.float32),
+                                       rng.standard_normal(h).astype(np.float32),
+                                       rng.standard_normal((h, h)).astype(np.float32)), m, 5000 + i, G, beta,
+                         2 * h * m * steps))
+    return jobs
+
+
+def run_serve(rocm, n_jobs, load, seed=20240601):
+    """Serving with continuous batching (sqaod_amd.rocm.Server): jobs are
+    admitted in arrival order under a device-memory budget while others run,
+    every tick steps all active jobs together (dense jobs in shared launches,
+    bipartite jobs on parallel streams).  First all jobs are submitted at once
+    to measure the server's capacity (jobs/s at saturation); then the same
+    jobs arrive as a Poisson process at `load` times that rate, and latency
+    = finish - arrival on the server's clock."""
+    jobs = serve_jobs(n_jobs, seed)
+
+    def submit(srv, j):
+        kind, data, m, sd, G, beta, _ = j
         if kind == "dense":
-            an = rocm.dense_graph_annealer(data[0], dtype=np.float32, n_trotters=m)
-            per = N * m
-        else:
-            an = rocm.bipartite_graph_annealer(*data, dtype=np.float32, n_trotters=m)
-            per = m * 2 * (N // 2)
-        an.seed(7)
-        an.prepare()
-        an.randomize_spin()
-        for k in range(steps):
-            an.anneal_one_step(schedule_G(k), BETA)
-        an.get_E()
-        dev.synchronize()
-        service.append(time.perf_counter() - t0)
-        attempts += per * steps
-        del an
-    service = np.array(service)
-    rate = load / service.mean()  # arrivals per second
+            return srv.submit_dense(data[0], m, sd, G, beta)
+        return srv.submit_bipartite(*data, m, sd, G, beta)
+
+    srv = rocm.server()
+    t0 = srv.now()
+    for j in jobs:
+        submit(srv, j)
+    ticks = 0
+    while srv.tick() > 0:
+        ticks += 1
+    makespan = srv.now() - t0
+    done = srv.take_done()
+    for d in done:
+        srv.release(d)
+    attempts = sum(j[-1] for j in jobs)
+    capacity = n_jobs / makespan
+    # Poisson arrivals at load x capacity
+    rate = load * capacity
+    rng = np.random.default_rng(seed + 1)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
-    done, lat = 0.0, []
-    for a, sv in zip(arrivals, service):
-        start = max(a, done)
-        done = start + sv
-        lat.append(done - a)
+    srv = rocm.server()
+    start = srv.now()
+    nxt, lat = 0, []
+
+    def collect():
+        for d in srv.take_done():
+            r = srv.result(d)
+            lat.append(r["finish_s"] - r["submit_s"])
+            srv.release(d)
+
+    while True:
+        now = srv.now() - start
+        while nxt < n_jobs and arrivals[nxt] <= now:
+            submit(srv, jobs[nxt])
+            nxt += 1
+        st = srv.status()
+        if st["pending"] + st["active"] == 0:
+            if nxt >= n_jobs:
+                break
+            time.sleep(max(0.0, min(arrivals[nxt] - now, 0.05)))
+            continue
+        srv.tick()
+        collect()
+    collect()
     lat = np.array(lat) * 1e3
-    return {"jobs": n_jobs, "qps": rate, "load": load, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "service_ms_mean": 1e3 * float(service.mean()),
-            "attempts_per_s": attempts / float(service.sum()),
-            "note": "FIFO single-stream server; latencies replayed over measured service times at Poisson "
-                    "arrivals of the stated rate"}
+    return {"jobs": n_jobs, "dense_jobs": sum(j[0] == "dense" for j in jobs),
+            "saturation_makespan_s": makespan, "saturation_ticks": ticks,
+            "capacity_jobs_per_s": capacity, "attempts_per_s": attempts / makespan,
+            "qps": rate, "load": load, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+            "note": "continuous batching: admission in arrival order under the memory budget, one tick steps "
+                    "every active job; attempts_per_s at saturation (all jobs submitted at once)"}
 
 
 def run_batch(rocm, n_inst, steps, seed=4242):

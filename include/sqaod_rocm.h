@@ -36,6 +36,7 @@ typedef struct sqaod_rocm_dg_bf_searcher sqaod_rocm_dg_bf_searcher;
 typedef struct sqaod_rocm_bg_annealer sqaod_rocm_bg_annealer;
 typedef struct sqaod_rocm_formulas sqaod_rocm_formulas;
 typedef struct sqaod_rocm_dense_batch sqaod_rocm_dense_batch;
+typedef struct sqaod_rocm_server sqaod_rocm_server;
 
 /* ---- library ---- */
 
@@ -287,6 +288,40 @@ int sqaod_rocm_dense_batch_get_shape(sqaod_rocm_dense_batch *b, int i, int *N, i
 int sqaod_rocm_dense_batch_get_E(sqaod_rocm_dense_batch *b, int i, float *E, int m);
 int sqaod_rocm_dense_batch_get_x(sqaod_rocm_dense_batch *b, int i, int8_t *x, int m, int N);
 int sqaod_rocm_dense_batch_get_q(sqaod_rocm_dense_batch *b, int i, int8_t *q, int m, int N);
+
+/* ---------------- serving (no counterpart in the reference) ----------------
+ * For callers that serve anneal requests as they arrive: jobs (dense or
+ * bipartite float32 QUBOs, each with its own seed and G/beta schedule) are
+ * admitted in arrival order while an active-job limit and a device-memory
+ * budget allow; every tick advances all active jobs by one step together
+ * (dense jobs in shared launches, bipartite jobs on parallel streams) and
+ * finishes the jobs whose schedule is done.  A job's E and x equal the same
+ * job run alone with the same seed and schedule. */
+int sqaod_rocm_server_new(sqaod_rocm_server **out);
+int sqaod_rocm_server_delete(sqaod_rocm_server *s);
+/* memory_budget_mb, max_active, dense_field_precision (fp32 | bf16x2 | fp8),
+ * bipartite_field_precision (fp32 | bf16x2), bipartite_streams, block_fraction */
+int sqaod_rocm_server_set_preference(sqaod_rocm_server *s, const char *name, const char *value);
+/* W: N x N symmetric (dense) or N1 x N0 (bipartite, b0[N0], b1[N1]); minimized */
+int sqaod_rocm_server_submit_dense(sqaod_rocm_server *s, const float *W, int N, int m, unsigned long long seed,
+                                   const double *G, const double *beta, int nSteps, int *id);
+int sqaod_rocm_server_submit_bipartite(sqaod_rocm_server *s, const float *b0, const float *b1, const float *W, int N0,
+                                       int N1, int m, unsigned long long seed, const double *G, const double *beta,
+                                       int nSteps, int *id);
+/* one step of every active job; *remaining = active + pending jobs after it */
+int sqaod_rocm_server_tick(sqaod_rocm_server *s, int *remaining);
+/* pending, active, bytes in use and the peak, by the admission estimate */
+int sqaod_rocm_server_status(sqaod_rocm_server *s, int *pending, int *active, unsigned long long *bytesInUse,
+                             unsigned long long *peakBytes);
+/* ids of the jobs finished since the last call (up to cap; *n written) */
+int sqaod_rocm_server_take_done(sqaod_rocm_server *s, int *ids, int cap, int *n);
+int sqaod_rocm_server_result_shape(sqaod_rocm_server *s, int id, int *kind, int *N0, int *N1, int *m);
+/* E[m]; x0[m x N0] (dense: x), x1[m x N1] (bipartite, else may be null);
+ * times: submit, admit, finish (seconds on the server clock); attempts of the job */
+int sqaod_rocm_server_result(sqaod_rocm_server *s, int id, float *E, int8_t *x0, int8_t *x1, double times[3],
+                             long long *attempts);
+int sqaod_rocm_server_release(sqaod_rocm_server *s, int id);
+int sqaod_rocm_server_now(sqaod_rocm_server *s, double *t);
 
 /* Self check without a reference counterpart: the fp8 sweep's e4m3 decode
  * (gfx950 v_cvt_pk_f32_fp8) applied to the byte codes 0..255 (out[256]). */

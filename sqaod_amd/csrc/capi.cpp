@@ -4,6 +4,7 @@
  * catches, records the message for sqaod_rocm_last_error() and returns -1,
  * the reference's TRY/CATCH_ERROR_AND_RETURN pattern (pyglue.h:394-399).
  */
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -504,6 +505,121 @@ int sqaod_rocm_dense_batch_get_x(sqaod_rocm_dense_batch *b, int i, int8_t *x, in
 
 int sqaod_rocm_dense_batch_get_q(sqaod_rocm_dense_batch *b, int i, int8_t *q, int m, int N) {
     return guarded([&] { batchInstance(b, i, m, N).get_q(q); });
+}
+
+/* ---------------- serving ---------------- */
+
+namespace {
+Server &serverOf(sqaod_rocm_server *s) {
+    SQ_THROW_IF(s == nullptr, "null server handle");
+    return *reinterpret_cast<Server *>(s);
+}
+}  // namespace
+
+int sqaod_rocm_server_new(sqaod_rocm_server **out) {
+    return guarded([&] {
+        SQ_THROW_IF(!out, "null output");
+        *out = reinterpret_cast<sqaod_rocm_server *>(new Server(defaultDevice()));
+    });
+}
+
+int sqaod_rocm_server_delete(sqaod_rocm_server *s) {
+    return guarded([&] { delete &serverOf(s); });
+}
+
+int sqaod_rocm_server_set_preference(sqaod_rocm_server *s, const char *name, const char *value) {
+    return guarded([&] {
+        SQ_THROW_IF(!name || !value, "null preference");
+        serverOf(s).setPreference(name, value);
+    });
+}
+
+int sqaod_rocm_server_submit_dense(sqaod_rocm_server *s, const float *W, int N, int m, unsigned long long seed,
+                                   const double *G, const double *beta, int nSteps, int *id) {
+    return guarded([&] {
+        SQ_THROW_IF(!id, "null id");
+        *id = serverOf(s).submitDense(W, N, m, seed, G, beta, nSteps);
+    });
+}
+
+int sqaod_rocm_server_submit_bipartite(sqaod_rocm_server *s, const float *b0, const float *b1, const float *W, int N0,
+                                       int N1, int m, unsigned long long seed, const double *G, const double *beta,
+                                       int nSteps, int *id) {
+    return guarded([&] {
+        SQ_THROW_IF(!id, "null id");
+        *id = serverOf(s).submitBipartite(b0, b1, W, N0, N1, m, seed, G, beta, nSteps);
+    });
+}
+
+int sqaod_rocm_server_tick(sqaod_rocm_server *s, int *remaining) {
+    return guarded([&] {
+        const int r = serverOf(s).tick();
+        if (remaining)
+            *remaining = r;
+    });
+}
+
+int sqaod_rocm_server_status(sqaod_rocm_server *s, int *pending, int *active, unsigned long long *bytesInUse,
+                             unsigned long long *peakBytes) {
+    return guarded([&] {
+        Server &sv = serverOf(s);
+        if (pending)
+            *pending = sv.numPending();
+        if (active)
+            *active = sv.numActive();
+        if (bytesInUse)
+            *bytesInUse = sv.memoryInUse();
+        if (peakBytes)
+            *peakBytes = sv.peakMemory();
+    });
+}
+
+int sqaod_rocm_server_take_done(sqaod_rocm_server *s, int *ids, int cap, int *n) {
+    return guarded([&] {
+        SQ_THROW_IF(!ids || !n || cap < 0, "bad output");
+        std::vector<int> d = serverOf(s).takeDone();
+        SQ_THROW_IF((int)d.size() > cap, "%zu finished jobs, room for %d", d.size(), cap);
+        std::copy(d.begin(), d.end(), ids);
+        *n = (int)d.size();
+    });
+}
+
+int sqaod_rocm_server_result_shape(sqaod_rocm_server *s, int id, int *kind, int *N0, int *N1, int *m) {
+    return guarded([&] {
+        const Server::Result &r = serverOf(s).result(id);
+        *kind = r.kind;
+        *N0 = r.N0;
+        *N1 = r.N1;
+        *m = r.m;
+    });
+}
+
+int sqaod_rocm_server_result(sqaod_rocm_server *s, int id, float *E, int8_t *x0, int8_t *x1, double times[3],
+                             long long *attempts) {
+    return guarded([&] {
+        const Server::Result &r = serverOf(s).result(id);
+        if (E)
+            std::copy(r.E.begin(), r.E.end(), E);
+        if (x0)
+            std::copy(r.x0.begin(), r.x0.end(), x0);
+        if (x1)
+            std::copy(r.x1.begin(), r.x1.end(), x1);
+        if (times) {
+            times[0] = r.submitS;
+            times[1] = r.admitS;
+            times[2] = r.finishS;
+        }
+        if (attempts)
+            *attempts = r.attempts;
+    });
+}
+
+int sqaod_rocm_server_release(sqaod_rocm_server *s, int id) {
+    return guarded([&] { serverOf(s).release(id); });
+}
+
+int sqaod_rocm_server_now(sqaod_rocm_server *s, double *t) {
+    return guarded([&] { *t = serverOf(s).now(); });
 }
 
 int sqaod_rocm_fp8_decode_table(float *out) {

@@ -16,25 +16,12 @@
  * directly or captured into one hipGraph and launched once.
  */
 #include <algorithm>
-#include <chrono>
 #include <cstring>
 #include <numeric>
 
 #include "solver.h"
 
 namespace sqaod_rocm {
-
-namespace {
-
-size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-/* offsets of one step's arrays inside the uploaded plan */
-struct StepLayout {
-    size_t gemmJobs, gemmStart, streamJobs, streamStart, args;
-    int nGemm, gemmBlocks, streamBlocks;
-};
-
-}  // namespace
 
 DenseBatch::DenseBatch(Device &dev) : dev_(&dev) {}
 
@@ -126,29 +113,20 @@ long long DenseBatch::attemptsPerStep() const {
 }
 
 void DenseBatch::buildItems() {
-    std::vector<int> order(inst_.size());
-    std::iota(order.begin(), order.end(), 0);
+    order_.resize(inst_.size());
+    std::iota(order_.begin(), order_.end(), 0);
     if (largestFirst_)
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        std::stable_sort(order_.begin(), order_.end(), [&](int x, int y) {
             const auto &a = *inst_[x], &b = *inst_[y];
             return a.getProblemSize() != b.getProblemSize() ? a.getProblemSize() > b.getProblemSize()
                                                             : a.numTrotters() > b.numTrotters();
         });
-    std::vector<uint2> items;
     maxNp_ = maxLd8_ = maxM_ = 0;
-    for (int i : order) {
-        auto &a = *inst_[i];
-        SQ_THROW_IF(a.fieldFormat() != fieldFormat_, "instance %d has another field format than the batch", i);
-        for (int y = 0; y < a.numTrotters(); ++y)
-            items.push_back(uint2{(unsigned)i, (unsigned)y});
-        maxNp_ = std::max(maxNp_, a.paddedN());
-        maxLd8_ = std::max(maxLd8_, a.fp8RowStrideBytes());
-        maxM_ = std::max(maxM_, a.numTrotters());
+    for (auto *a : inst_) {
+        maxNp_ = std::max(maxNp_, a->paddedN());
+        maxLd8_ = std::max(maxLd8_, a->fp8RowStrideBytes());
+        maxM_ = std::max(maxM_, a->numTrotters());
     }
-    nItems_ = (int)items.size();
-    d_items_.allocate(items.size());
-    HIP_CHECK(hipMemcpy(d_items_.ptr, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    d_queue_.allocate(1);
     itemsValid_ = true;
 }
 
@@ -158,42 +136,80 @@ void DenseBatch::anneal(const double *G, const double *beta, int nSteps, bool gr
 
 void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph, int wpb) {
     SQ_THROW_IF(inst_.empty(), "the batch holds no instances");
-    SQ_THROW_IF(nSteps <= 0, "no steps");
-    dev_->makeCurrent();
     if (!itemsValid_)
         buildItems();
-    const int nI = (int)inst_.size();
-    for (auto *a : inst_) {
-        a->throwErrorIfQNotSet();
-        a->checkStepAbort();
-        a->reserveTags(nSteps);
+    GroupedDenseStepper::Options o;
+    o.fieldFormat = fieldFormat_;
+    o.wavesPerBlock = wpb;
+    o.graph = graph;
+    stepper_.run(*dev_, inst_, order_, nSteps, [&](int k, int) { return std::make_pair(G[k], beta[k]); }, o);
+    lastMs_ = stepper_.lastMs();
+}
+
+/* ---------------- GroupedDenseStepper ---------------- */
+
+void GroupedDenseStepper::run(Device &dev, const std::vector<DenseGraphAnnealer<float> *> &inst,
+                              const std::vector<int> &order, int nSteps, const Schedule &sched, const Options &o) {
+    SQ_THROW_IF(inst.empty() || order.empty(), "grouped step: no instances");
+    SQ_THROW_IF(nSteps <= 0, "no steps");
+    dev.makeCurrent();
+    const int nI = (int)inst.size();
+    int maxNp = 0, maxLd8 = 0, maxM = 0;
+    std::vector<uint2> items;
+    for (int i : order) {
+        SQ_THROW_IF(i < 0 || i >= nI, "grouped step: bad instance order");
+        auto &a = *inst[i];
+        SQ_THROW_IF(a.fieldFormat() != o.fieldFormat, "instance %d has another field format than the group", i);
+        for (int y = 0; y < a.numTrotters(); ++y)
+            items.push_back(uint2{(unsigned)i, (unsigned)y});
+        maxNp = std::max(maxNp, a.paddedN());
+        maxLd8 = std::max(maxLd8, a.fp8RowStrideBytes());
+        maxM = std::max(maxM, a.numTrotters());
     }
-    const size_t regionBytes = denseSweepBatchRegionBytes(maxNp_, maxLd8_, sizeof(float), false, fieldFormat_);
-    const size_t lds = regionBytes * wpb + 256;
-    SQ_THROW_IF(lds > dev_->maxLdsPerBlock(), "batch: N=%d too large for the LDS-resident sweep", maxNp_);
-    const int capacity = denseSweepBatchBlocks<float>(regionBytes, wpb, false, fieldFormat_, dev_->numCUs());
-    SQ_THROW_IF((long long)capacity * wpb < maxM_,
-                "batch: %d resident waves cannot hold an instance of %d slices", capacity * wpb, maxM_);
-    const int nBlocks = std::min(capacity, (nItems_ + wpb - 1) / wpb);
+    const int nItems = (int)items.size();
+    std::vector<char> member(nI, 0);
+    for (int i : order)
+        member[i] = 1;
+    for (int i : order) {
+        inst[i]->throwErrorIfQNotSet();
+        inst[i]->checkStepAbort();
+        inst[i]->reserveTags(nSteps);
+    }
+    const int wpb = o.wavesPerBlock;
+    const size_t regionBytes = denseSweepBatchRegionBytes(maxNp, maxLd8, sizeof(float), false, o.fieldFormat);
+    SQ_THROW_IF(regionBytes * wpb + 256 > dev.maxLdsPerBlock(), "grouped step: N=%d too large for the LDS-resident sweep",
+                maxNp);
+    int capacity = denseSweepBatchBlocks<float>(regionBytes, wpb, false, o.fieldFormat, dev.numCUs());
+    if (o.blockFraction > 0. && o.blockFraction < 1.)
+        capacity = std::max(1, (int)(capacity * o.blockFraction));
+    SQ_THROW_IF((long long)capacity * wpb < maxM, "grouped step: %d resident waves cannot hold an instance of %d slices",
+                capacity * wpb, maxM);
+    const int nBlocks = std::min(capacity, (nItems + wpb - 1) / wpb);
 
     /* plan every step on the host, in the order annealOneStep takes them */
+    struct StepLayout {
+        size_t gj, gs, sj, ss, sa;
+        int nGemm, gemmBlocks, streamBlocks;
+    };
     std::vector<StepLayout> lay(nSteps);
     std::vector<FieldGemmJob> gj;
-    std::vector<int> gs;
+    std::vector<int> gs, ss;
     std::vector<AttemptStreamJob> sj;
-    std::vector<int> ss;
     std::vector<SweepArgs<float>> sa;
-    std::vector<size_t> gjOff(nSteps), sjOff(nSteps), saOff(nSteps), gsOff(nSteps), ssOff(nSteps);
     for (int k = 0; k < nSteps; ++k) {
         StepLayout &L = lay[k];
-        gjOff[k] = gj.size();
-        gsOff[k] = gs.size();
-        sjOff[k] = sj.size();
-        ssOff[k] = ss.size();
-        saOff[k] = sa.size();
+        L.gj = gj.size();
+        L.gs = gs.size();
+        L.sj = sj.size();
+        L.ss = ss.size();
+        L.sa = sa.size();
         int gBlocks = 0, sBlocks = 0;
         for (int i = 0; i < nI; ++i) {
-            auto &a = *inst_[i];
+            auto &a = *inst[i];
+            if (!member[i]) {
+                sa.push_back(SweepArgs<float>{}); /* never referenced by an item */
+                continue;
+            }
             if (a.refreshDue()) {
                 FieldGemmJob j = a.takeFieldGemmJob();
                 gs.push_back(gBlocks);
@@ -204,29 +220,37 @@ void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph
             ss.push_back(sBlocks);
             sBlocks += (int)(((long long)s.N * s.m + kAttemptBlock - 1) / kAttemptBlock);
             sj.push_back(s);
-            sa.push_back(a.makeSweepArgs((float)G[k], (float)beta[k]));
+            const auto gb = sched(k, i);
+            sa.push_back(a.makeSweepArgs((float)gb.first, (float)gb.second));
             a.stepDone();
         }
-        L.nGemm = (int)(gj.size() - gjOff[k]);
+        L.nGemm = (int)(gj.size() - L.gj);
         L.gemmBlocks = gBlocks;
         L.streamBlocks = sBlocks;
     }
+    const int nStream = (int)(sj.size() / nSteps);
+
     /* one pinned buffer, one upload */
+    auto alignUp = [](size_t v) { return (v + 255) / 256 * 256; };
     size_t off = 0;
+    const size_t oIT = off;
+    off = alignUp(off + items.size() * sizeof(uint2));
     const size_t oGJ = off;
-    off = alignUp(off + gj.size() * sizeof(FieldGemmJob), 256);
+    off = alignUp(off + gj.size() * sizeof(FieldGemmJob));
     const size_t oGS = off;
-    off = alignUp(off + gs.size() * sizeof(int), 256);
+    off = alignUp(off + gs.size() * sizeof(int));
     const size_t oSJ = off;
-    off = alignUp(off + sj.size() * sizeof(AttemptStreamJob), 256);
+    off = alignUp(off + sj.size() * sizeof(AttemptStreamJob));
     const size_t oSS = off;
-    off = alignUp(off + ss.size() * sizeof(int), 256);
+    off = alignUp(off + ss.size() * sizeof(int));
     const size_t oSA = off;
-    off = alignUp(off + sa.size() * sizeof(SweepArgs<float>), 256);
+    off = alignUp(off + sa.size() * sizeof(SweepArgs<float>));
     h_plan_.allocate(off);
     if (d_plan_.n < off)
         d_plan_.allocate(off);
+    d_queue_.allocate(1);
     unsigned char *h = h_plan_.ptr;
+    memcpy(h + oIT, items.data(), items.size() * sizeof(uint2));
     if (!gj.empty())
         memcpy(h + oGJ, gj.data(), gj.size() * sizeof(FieldGemmJob));
     if (!gs.empty())
@@ -234,7 +258,7 @@ void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph
     memcpy(h + oSJ, sj.data(), sj.size() * sizeof(AttemptStreamJob));
     memcpy(h + oSS, ss.data(), ss.size() * sizeof(int));
     memcpy(h + oSA, sa.data(), sa.size() * sizeof(SweepArgs<float>));
-    hipStream_t st = dev_->stream();
+    hipStream_t st = dev.stream();
     HIP_CHECK(hipMemcpyAsync(d_plan_.ptr, h, off, hipMemcpyHostToDevice, st));
     unsigned char *d = d_plan_.ptr;
 
@@ -242,12 +266,13 @@ void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph
         for (int k = 0; k < nSteps; ++k) {
             const StepLayout &L = lay[k];
             if (L.nGemm > 0)
-                fieldRefreshGrouped(fieldFormat_, reinterpret_cast<const FieldGemmJob *>(d + oGJ) + gjOff[k],
-                                    reinterpret_cast<const int *>(d + oGS) + gsOff[k], L.nGemm, L.gemmBlocks, st);
-            philoxAttemptsGrouped<float>(reinterpret_cast<const AttemptStreamJob *>(d + oSJ) + sjOff[k],
-                                         reinterpret_cast<const int *>(d + oSS) + ssOff[k], nI, L.streamBlocks, st);
-            denseSweepBatch<float>(reinterpret_cast<const SweepArgs<float> *>(d + oSA) + saOff[k], d_items_.ptr,
-                                   nItems_, d_queue_.ptr, false, fieldFormat_, regionBytes, wpb, nBlocks, st);
+                fieldRefreshGrouped(o.fieldFormat, reinterpret_cast<const FieldGemmJob *>(d + oGJ) + L.gj,
+                                    reinterpret_cast<const int *>(d + oGS) + L.gs, L.nGemm, L.gemmBlocks, st);
+            philoxAttemptsGrouped<float>(reinterpret_cast<const AttemptStreamJob *>(d + oSJ) + L.sj,
+                                         reinterpret_cast<const int *>(d + oSS) + L.ss, nStream, L.streamBlocks, st);
+            denseSweepBatch<float>(reinterpret_cast<const SweepArgs<float> *>(d + oSA) + L.sa,
+                                   reinterpret_cast<const uint2 *>(d + oIT), nItems, d_queue_.ptr, false,
+                                   o.fieldFormat, regionBytes, wpb, nBlocks, st);
         }
     };
 
@@ -256,8 +281,16 @@ void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph
     HIP_CHECK(hipEventCreate(&e1));
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
+    auto cleanup = [&]() {
+        if (ge)
+            (void)hipGraphExecDestroy(ge);
+        if (g)
+            (void)hipGraphDestroy(g);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    };
     try {
-        if (graph) {
+        if (o.graph) {
             HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
             try {
                 launchAll();
@@ -275,27 +308,21 @@ void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph
             launchAll();
             HIP_CHECK(hipEventRecord(e1, st));
         }
-        HIP_CHECK(hipStreamSynchronize(st));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        lastMs_ = ms;
+        if (o.wait) {
+            HIP_CHECK(hipStreamSynchronize(st));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            lastMs_ = ms;
+        }
     } catch (...) {
-        if (ge)
-            (void)hipGraphExecDestroy(ge);
-        if (g)
-            (void)hipGraphDestroy(g);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        cleanup();
         throw;
     }
-    if (ge)
-        HIP_CHECK(hipGraphExecDestroy(ge));
-    if (g)
-        HIP_CHECK(hipGraphDestroy(g));
-    HIP_CHECK(hipEventDestroy(e0));
-    HIP_CHECK(hipEventDestroy(e1));
-    for (auto *a : inst_)
-        a->checkStepAbort();
+    if (!o.wait)
+        HIP_CHECK(hipStreamSynchronize(st)); /* the plan buffer and graph are released below */
+    cleanup();
+    for (int i : order)
+        inst[i]->checkStepAbort();
 }
 
 void DenseBatch::autotune(double G, double beta, int trialSteps) {

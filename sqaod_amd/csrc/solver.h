@@ -10,7 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -261,6 +263,30 @@ private:
  * (instance, slice), optionally captured as one hipGraph for the whole
  * schedule.  Each instance is a DenseGraphAnnealer<float> and its results
  * are bit-identical to running it alone with the same seed and preferences. */
+class GroupedDenseStepper {
+public:
+    /* (G, beta) of step k for instance i */
+    using Schedule = std::function<std::pair<double, double>(int, int)>;
+    struct Options {
+        int fieldFormat = fieldF32;
+        int wavesPerBlock = 1;
+        bool graph = false;
+        bool wait = true;            /* synchronise and time the steps */
+        double blockFraction = 1.0;  /* share of the resident blocks the persistent sweep may use */
+    };
+    /* nSteps steps of the instances listed in `order` (items of the
+     * persistent sweep in that order); the others are left untouched */
+    void run(Device &dev, const std::vector<DenseGraphAnnealer<float> *> &inst, const std::vector<int> &order,
+             int nSteps, const Schedule &sched, const Options &o);
+    double lastMs() const { return lastMs_; }
+
+private:
+    DevBuf<unsigned char> d_plan_;
+    PinnedBuf<unsigned char> h_plan_;
+    DevBuf<int> d_queue_;
+    double lastMs_ = 0.;
+};
+
 class DenseBatch {
 public:
     explicit DenseBatch(Device &dev);
@@ -299,11 +325,9 @@ private:
     bool largestFirst_ = true;
     bool itemsValid_ = false;
     double lastMs_ = 0.;
-    DevBuf<uint2> d_items_;
-    DevBuf<int> d_queue_;
-    DevBuf<unsigned char> d_plan_;
-    PinnedBuf<unsigned char> h_plan_;
-    int nItems_ = 0, maxNp_ = 0, maxLd8_ = 0, maxM_ = 0;
+    GroupedDenseStepper stepper_;
+    std::vector<int> order_;
+    int maxNp_ = 0, maxLd8_ = 0, maxM_ = 0;
 };
 
 /* ------------------------------------------------------------------ */
@@ -458,6 +482,71 @@ private:
     void run(QuadFormArgs<real> a, const real *A, size_t nA, const real *lin0, size_t n0, const real *lin1,
              size_t n1, const int8_t *U, size_t nU, const int8_t *V, size_t nV, real *E, int nOut);
     Device *dev_ = nullptr;
+};
+
+/* ------------------------------------------------------------------ */
+/* Serving: continuous batching of anneal jobs (no reference counterpart) */
+/* ------------------------------------------------------------------ */
+/* Jobs (dense or bipartite float32 problems, each with its own seed and
+ * G/beta schedule) are submitted at any time and admitted in arrival order
+ * while the active-job limit and the device-memory budget allow.  A tick
+ * advances every active job by one step: the dense jobs through grouped
+ * launches on the server's stream (one persistent sweep per size class),
+ * the bipartite jobs on a pool of other streams at the same time.  A job
+ * that has run its schedule is finished (makeSolution: FP64 energies, x),
+ * its device state is freed and its results wait to be collected.  Each
+ * job's results equal the same job run alone with the same seed. */
+class Server {
+public:
+    enum Kind { kDense = 0, kBipartite = 1 };
+    struct Result {
+        int kind, N0, N1, m;
+        std::vector<float> E;
+        std::vector<int8_t> x0, x1; /* dense: x0 is m x N */
+        double submitS, admitS, finishS; /* seconds on the server's clock */
+        long long attempts;
+    };
+    explicit Server(Device &dev);
+    ~Server();
+    /* memory_budget_mb, max_active, dense_field_precision, bipartite_field_precision,
+     * bipartite_streams, block_fraction */
+    void setPreference(const std::string &name, const std::string &value);
+    int submitDense(const float *W, int N, int m, unsigned long long seed, const double *G, const double *beta,
+                    int nSteps);
+    int submitBipartite(const float *b0, const float *b1, const float *W, int N0, int N1, int m,
+                        unsigned long long seed, const double *G, const double *beta, int nSteps);
+    /* admit, step every active job once, finish completed jobs; returns the
+     * number of jobs still active or pending */
+    int tick();
+    int numPending() const { return (int)pending_.size(); }
+    int numActive() const { return (int)active_.size(); }
+    std::vector<int> takeDone();
+    const Result &result(int id) const;
+    void release(int id);
+    size_t memoryInUse() const { return memInUse_; }
+    size_t peakMemory() const { return memPeak_; }
+    double now() const;
+    static size_t estimateDenseBytes(int N, int m, int fieldFormat);
+    static size_t estimateBipartiteBytes(int N0, int N1, int m, int fieldFormat);
+
+private:
+    struct Job;
+    void admit();
+    void finish(Job &j);
+
+    Device *dev_;
+    std::vector<std::unique_ptr<Device>> pool_;
+    size_t budget_ = (size_t)64 << 30;
+    int maxActive_ = 256;
+    int denseFormat_ = fieldBF16x2, bipartiteFormat_ = fieldBF16x2;
+    double blockFraction_ = 1.0;
+    int nextId_ = 0, nextStream_ = 0;
+    size_t memInUse_ = 0, memPeak_ = 0;
+    std::vector<std::unique_ptr<Job>> pending_, active_;
+    std::map<int, Result> done_;
+    std::vector<int> newlyDone_;
+    std::map<int, GroupedDenseStepper> steppers_; /* per size class */
+    double t0_;
 };
 
 }  // namespace sqaod_rocm

@@ -7,3 +7,4 @@ from .dense_graph_bf_searcher import dense_graph_bf_searcher, DenseGraphBFSearch
 from .bipartite_graph_annealer import bipartite_graph_annealer, BipartiteGraphAnnealer  # noqa: F401
 from . import formulas  # noqa: F401
 from .dense_batch import dense_batch, DenseBatch  # noqa: F401
+from .serve import server, Server  # noqa: F401

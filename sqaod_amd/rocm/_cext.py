@@ -123,6 +123,21 @@ _SIGS = {
     "sqaod_rocm_dense_batch_get_E": (_i32, [_vp, _i32, _vp, _i32]),
     "sqaod_rocm_dense_batch_get_x": (_i32, [_vp, _i32, _vp, _i32, _i32]),
     "sqaod_rocm_dense_batch_get_q": (_i32, [_vp, _i32, _vp, _i32, _i32]),
+    "sqaod_rocm_server_new": (_i32, [_PP]),
+    "sqaod_rocm_server_delete": (_i32, [_vp]),
+    "sqaod_rocm_server_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "sqaod_rocm_server_submit_dense": (_i32, [_vp, _vp, _i32, _i32, _u64, _vp, _vp, _i32, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_server_submit_bipartite": (_i32, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _u64, _vp, _vp, _i32,
+                                                  ctypes.POINTER(_i32)]),
+    "sqaod_rocm_server_tick": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_server_status": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_u64),
+                                        ctypes.POINTER(_u64)]),
+    "sqaod_rocm_server_take_done": (_i32, [_vp, _vp, _i32, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_server_result_shape": (_i32, [_vp, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                              ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaod_rocm_server_result": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "sqaod_rocm_server_release": (_i32, [_vp, _i32]),
+    "sqaod_rocm_server_now": (_i32, [_vp, ctypes.POINTER(_f64)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -746,9 +761,102 @@ class _DenseBatch:
         return q
 
 
+class _Server:
+    """ctypes front of sqaod_rocm_server_*."""
+
+    @staticmethod
+    def new():
+        p = _vp()
+        _check(_lib.sqaod_rocm_server_new(ctypes.byref(p)))
+        return p
+
+    @staticmethod
+    def delete(cobj):
+        _check(_lib.sqaod_rocm_server_delete(cobj))
+
+    @staticmethod
+    def set_preferences(cobj, prefs):
+        for k, v in prefs.items():
+            _check(_lib.sqaod_rocm_server_set_preference(cobj, str(k).encode(), str(v).encode()))
+
+    @staticmethod
+    def submit_dense(cobj, W, m, seed, G, beta):
+        W = np.ascontiguousarray(W, np.float32)
+        G = np.ascontiguousarray(G, np.float64)
+        beta = np.ascontiguousarray(np.broadcast_to(beta, G.shape), np.float64)
+        i = _i32()
+        _check(_lib.sqaod_rocm_server_submit_dense(cobj, _ptr(W), W.shape[0], int(m), int(seed), _ptr(G), _ptr(beta),
+                                                   G.shape[0], ctypes.byref(i)))
+        return i.value
+
+    @staticmethod
+    def submit_bipartite(cobj, b0, b1, W, m, seed, G, beta):
+        b0 = np.ascontiguousarray(b0, np.float32)
+        b1 = np.ascontiguousarray(b1, np.float32)
+        W = np.ascontiguousarray(W, np.float32)
+        G = np.ascontiguousarray(G, np.float64)
+        beta = np.ascontiguousarray(np.broadcast_to(beta, G.shape), np.float64)
+        i = _i32()
+        _check(_lib.sqaod_rocm_server_submit_bipartite(cobj, _ptr(b0), _ptr(b1), _ptr(W), b0.shape[0], b1.shape[0],
+                                                       int(m), int(seed), _ptr(G), _ptr(beta), G.shape[0],
+                                                       ctypes.byref(i)))
+        return i.value
+
+    @staticmethod
+    def tick(cobj):
+        r = _i32()
+        _check(_lib.sqaod_rocm_server_tick(cobj, ctypes.byref(r)))
+        return r.value
+
+    @staticmethod
+    def status(cobj):
+        p, a, b, pk = _i32(), _i32(), _u64(), _u64()
+        _check(_lib.sqaod_rocm_server_status(cobj, ctypes.byref(p), ctypes.byref(a), ctypes.byref(b),
+                                             ctypes.byref(pk)))
+        return {"pending": p.value, "active": a.value, "bytes_in_use": b.value, "peak_bytes": pk.value}
+
+    @staticmethod
+    def take_done(cobj, cap=4096):
+        ids = np.empty(cap, np.int32)
+        n = _i32()
+        _check(_lib.sqaod_rocm_server_take_done(cobj, _ptr(ids), cap, ctypes.byref(n)))
+        return [int(v) for v in ids[:n.value]]
+
+    @staticmethod
+    def result(cobj, i):
+        kind, N0, N1, m = _i32(), _i32(), _i32(), _i32()
+        _check(_lib.sqaod_rocm_server_result_shape(cobj, i, ctypes.byref(kind), ctypes.byref(N0), ctypes.byref(N1),
+                                                   ctypes.byref(m)))
+        E = np.empty(m.value, np.float32)
+        x0 = np.empty((m.value, N0.value), np.int8)
+        x1 = np.empty((m.value, N1.value), np.int8) if kind.value == 1 else None
+        t = np.empty(3, np.float64)
+        att = ctypes.c_longlong()
+        _check(_lib.sqaod_rocm_server_result(cobj, i, _ptr(E), _ptr(x0), _ptr(x1) if x1 is not None else None,
+                                             _ptr(t), ctypes.byref(att)))
+        out = {"kind": "dense" if kind.value == 0 else "bipartite", "E": E, "submit_s": t[0], "admit_s": t[1],
+               "finish_s": t[2], "attempts": att.value}
+        if kind.value == 0:
+            out["x"] = x0
+        else:
+            out["x0"], out["x1"] = x0, x1
+        return out
+
+    @staticmethod
+    def release(cobj, i):
+        _check(_lib.sqaod_rocm_server_release(cobj, i))
+
+    @staticmethod
+    def now(cobj):
+        t = _f64()
+        _check(_lib.sqaod_rocm_server_now(cobj, ctypes.byref(t)))
+        return t.value
+
+
 rocm_device = _Device
 rocm_dg_annealer = _DenseGraphAnnealer
 rocm_dg_bf_searcher = _DenseGraphBFSearcher
 rocm_bg_annealer = _BipartiteGraphAnnealer
 rocm_formulas = _Formulas
 rocm_dense_batch = _DenseBatch
+rocm_server = _Server
