@@ -163,6 +163,22 @@ def run_bf(rocm, N, seed=1133557):
             "n_solutions": len(s.get_x())}
 
 
+def run_bg_bf(rocm, N0, N1, seed=1133557):
+    """Bipartite brute-force search over all 2^(N0+N1) pairs, b0, b1, W
+    integers in [-8, 8]."""
+    rng = np.random.default_rng(seed + N0 * 64 + N1)
+    b0 = rng.integers(-8, 9, N0).astype(np.float32)
+    b1 = rng.integers(-8, 9, N1).astype(np.float32)
+    W = rng.integers(-8, 9, (N1, N0)).astype(np.float32)
+    s = rocm.bipartite_graph_bf_searcher(b0, b1, W, dtype=np.float32)
+    s.search()  # warm-up
+    t0 = time.perf_counter()
+    s.search()
+    el = time.perf_counter() - t0
+    return {"N0": N0, "N1": N1, "pairs_per_s": float(2 ** (N0 + N1)) / el, "ms": 1e3 * el,
+            "E_min": float(s.get_E()[0]), "n_solutions": len(s.get_x())}
+
+
 def serve_jobs(n_jobs, seed):
     """70% dense / 30% bipartite (N0 = N1 = N/2) jobs, N log-uniform in
     [32, 8192], m uniform in {16..256}, weights N(0,1), each with its own
@@ -419,6 +435,9 @@ def main():
         print(f"# batch: {out['batch']['attempts_per_s']:.4g} attempts/s over {args.batch_instances} instances",
               file=sys.stderr, flush=True)
         out["bf_search"] = [run_bf(rocm, n) for n in (24, 30)]
+        out["bipartite_bf_search"] = [run_bg_bf(rocm, 12, 12), run_bg_bf(rocm, 16, 16)]
+        print(f"# bipartite bf search 16x16: {out['bipartite_bf_search'][1]['pairs_per_s']:.4g} pairs/s",
+              file=sys.stderr, flush=True)
         print(f"# bf search N=30: {out['bf_search'][1]['candidates_per_s']:.4g} candidates/s", file=sys.stderr,
               flush=True)
         out["serve"] = run_serve(rocm, args.serve_jobs, 0.7)

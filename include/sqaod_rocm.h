@@ -34,6 +34,7 @@ typedef struct sqaod_rocm_device sqaod_rocm_device;
 typedef struct sqaod_rocm_dg_annealer sqaod_rocm_dg_annealer;
 typedef struct sqaod_rocm_dg_bf_searcher sqaod_rocm_dg_bf_searcher;
 typedef struct sqaod_rocm_bg_annealer sqaod_rocm_bg_annealer;
+typedef struct sqaod_rocm_bg_bf_searcher sqaod_rocm_bg_bf_searcher;
 typedef struct sqaod_rocm_formulas sqaod_rocm_formulas;
 typedef struct sqaod_rocm_dense_batch sqaod_rocm_dense_batch;
 typedef struct sqaod_rocm_server sqaod_rocm_server;
@@ -163,6 +164,38 @@ int sqaod_rocm_dg_bf_searcher_get_n_solutions(sqaod_rocm_dg_bf_searcher *b, int 
 int sqaod_rocm_dg_bf_searcher_get_x(sqaod_rocm_dg_bf_searcher *b, int8_t *x, int n, int N);
 /* bf_searcher_get_E (:318-341): E[n] of dtype, n = max(1, number of solutions) */
 int sqaod_rocm_dg_bf_searcher_get_E(sqaod_rocm_dg_bf_searcher *b, void *E, int n);
+
+/* ---- bipartite graph brute-force searcher: bf_searcher.inc method table
+ * (:479-494), the BIPARTITE_GRAPH variants.  W is N1 x N0.  Result: Emin = min
+ * over all (x0, x1) of b0.x0 + b1.x1 + x1^T W x0 (maximize: max), and the
+ * first N0 + N1 pairs reaching it in one worker's visiting order (x0 tile of
+ * tile_size_0, then x1, then x0 within the tile;
+ * CPUBipartiteGraphBatchSearch.cpp:29-62, CPUBipartiteGraphBFSearcher.cpp:
+ * 166-198). ---- */
+int sqaod_rocm_bg_bf_searcher_new(int dtype, sqaod_rocm_bg_bf_searcher **out);
+int sqaod_rocm_bg_bf_searcher_delete(sqaod_rocm_bg_bf_searcher *b);
+int sqaod_rocm_bg_bf_searcher_assign_device(sqaod_rocm_bg_bf_searcher *b, sqaod_rocm_device *dev);
+/* bf_searcher_set_qubo / get_problem_size (:133-192): b0[N0], b1[N1], W[N1 x N0], N0, N1 <= 63 */
+int sqaod_rocm_bg_bf_searcher_set_qubo(sqaod_rocm_bg_bf_searcher *b, const void *b0, int N0, const void *b1, int N1,
+                                       const void *W, int optimize);
+int sqaod_rocm_bg_bf_searcher_get_problem_size(sqaod_rocm_bg_bf_searcher *b, int *N0, int *N1);
+/* "tile_size_0", "tile_size_1" (rounded up to multiples of 256, Solver.cpp:231-251) */
+int sqaod_rocm_bg_bf_searcher_set_preference(sqaod_rocm_bg_bf_searcher *b, const char *name, const char *value);
+int sqaod_rocm_bg_bf_searcher_get_preferences(sqaod_rocm_bg_bf_searcher *b, char *buf, size_t buflen,
+                                              size_t *needed);
+int sqaod_rocm_bg_bf_searcher_prepare(sqaod_rocm_bg_bf_searcher *b);
+int sqaod_rocm_bg_bf_searcher_calculate_E(sqaod_rocm_bg_bf_searcher *b);
+int sqaod_rocm_bg_bf_searcher_make_solution(sqaod_rocm_bg_bf_searcher *b);
+/* bf_searcher_search_range (:429-451): scans the next rectangle of pairs;
+ * *done = 1 once all are scanned; *cur_x0, *cur_x1: the position reached */
+int sqaod_rocm_bg_bf_searcher_search_range(sqaod_rocm_bg_bf_searcher *b, int *done, unsigned long long *cur_x0,
+                                           unsigned long long *cur_x1);
+int sqaod_rocm_bg_bf_searcher_search(sqaod_rocm_bg_bf_searcher *b);
+int sqaod_rocm_bg_bf_searcher_get_n_solutions(sqaod_rocm_bg_bf_searcher *b, int *n);
+/* bf_searcher_get_x (:277-319): x0[n x N0], x1[n x N1] int8 */
+int sqaod_rocm_bg_bf_searcher_get_x(sqaod_rocm_bg_bf_searcher *b, int8_t *x0, int8_t *x1, int n, int N0, int N1);
+/* bf_searcher_get_E (:321-346): E[max(1, n)] of dtype */
+int sqaod_rocm_bg_bf_searcher_get_E(sqaod_rocm_bg_bf_searcher *b, void *E, int n);
 
 /* ---- bipartite graph annealer: annealer.inc method table (:885-908), the
  * BIPARTITE_GRAPH variants.  W is N1 x N0 (rows index the x1 side), J likewise;

@@ -80,6 +80,7 @@ def lib():
             "sqo_bg_hamiltonian_f64": (None, [i32, i32, vp, vp, vp, vp, vp, vp, vp]),
             "sqo_bg_energy_q_f64acc": (None, [i32, i32, i32, vp, vp, vp, f64, vp, vp, vp]),
             "sqo_dg_bf_search": (i32, [i32, i32, vp, i32, i32, vp, vp]),
+            "sqo_bg_bf_search": (i32, [i32, i32, i32, vp, vp, vp, i32, i32, i32, vp, vp, vp]),
             "sqo_expf_checksum": (u64, [u64, u64]),
         }
         for name, (res, args) in sigs.items():
@@ -337,6 +338,23 @@ def dense_bf_search(W, dtype=np.float32, optimize=MINIMIZE, tile_size=1024):
     if n < 0:
         raise ValueError("N out of range")
     return Emin.value, xs[:n].copy()
+
+
+def bipartite_bf_search(b0, b1, W, dtype=np.float32, optimize=MINIMIZE, tile_size0=1024, tile_size1=1024):
+    """Returns (Emin, xs0, xs1): the packed minimising pairs in the one-worker
+    visiting order, at most N0 + N1 of them."""
+    b0 = np.ascontiguousarray(b0, np.float64)
+    b1 = np.ascontiguousarray(b1, np.float64)
+    W = np.ascontiguousarray(W, np.float64)
+    N0, N1 = len(b0), len(b1)
+    xs0 = np.empty(N0 + N1, np.uint64)
+    xs1 = np.empty(N0 + N1, np.uint64)
+    Emin = ctypes.c_double()
+    n = lib().sqo_bg_bf_search(_dt(dtype), N0, N1, _p(b0), _p(b1), _p(W), optimize, tile_size0, tile_size1,
+                               ctypes.byref(Emin), _p(xs0), _p(xs1))
+    if n < 0:
+        raise ValueError("N0, N1 out of range")
+    return Emin.value, xs0[:n].copy(), xs1[:n].copy()
 
 
 def unpack_bits(packed, N):

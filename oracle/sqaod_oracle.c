@@ -443,6 +443,73 @@ int sqo_dg_bf_search(int dtype, int N, const double *W, int om, int tileSize,
     return n;
 }
 
+/* ------------------------------------------------------------------ */
+/* bipartite brute-force search: CPUBipartiteGraphBFSearcher.cpp:75-225 */
+/* and CPUBipartiteGraphBatchSearch.cpp:29-62, single-worker semantics  */
+/* ------------------------------------------------------------------ */
+int sqo_bg_bf_search(int dtype, int N0, int N1, const double *b0, const double *b1, const double *W, int om,
+                     int tileSize0, int tileSize1, double *EminOut, unsigned long long *xs0,
+                     unsigned long long *xs1) {
+    /* E(x0, x1) = b0.x0 + b1.x1 + x1^T W x0 (calculate_E_2d, SharedFormulas.cpp:123-144; W is
+     * N1 x N0, negated with b0, b1 when maximizing, :39-47), bit (N-1-pos) of a packed x holding
+     * x[pos] (Common.cpp:79-86).  One worker visits x0 tiles in order, and inside each the x1
+     * tiles in order (searchRangeSingleThread, :166-198); a tile pair is scanned x1-major
+     * (BatchSearch :47-60), so pairs are met in the order (x0 / T0, x1, x0 % T0).  Strictly lower
+     * E restarts the list, equal E appends while it holds fewer than N0 + N1 pairs (:45, :52).
+     * Energies are exact in double and rounded to the solver's real type before comparison. */
+    if (N0 < 1 || N1 < 1 || N0 > 30 || N1 > 30 || N0 + N1 > 40)
+        return -1;
+    const unsigned long long x0max = 1ull << N0, x1max = 1ull << N1;
+    unsigned long long T0 = (unsigned long long)tileSize0;
+    if (T0 > x0max)
+        T0 = x0max; /* prepare, :78-85 */
+    (void)tileSize1; /* x1 tiles are visited in order, so they do not change the pair order */
+    const double sgn = (om == SQO_MAXIMIZE) ? -1. : 1.;
+    const int maxN = N0 + N1;
+    double Emin = FLT_MAX;
+    int n = 0;
+    for (unsigned long long t0 = 0; t0 < x0max; t0 += T0) {
+        for (unsigned long long x1 = 0; x1 < x1max; ++x1) {
+            double bx1 = 0.;
+            for (int i = 0; i < N1; ++i)
+                if ((x1 >> (N1 - 1 - i)) & 1ull)
+                    bx1 += sgn * b1[i];
+            for (unsigned long long x0 = t0; x0 < t0 + T0; ++x0) {
+                double E = bx1;
+                for (int j = 0; j < N0; ++j)
+                    if ((x0 >> (N0 - 1 - j)) & 1ull)
+                        E += sgn * b0[j];
+                for (int i = 0; i < N1; ++i) {
+                    if (!((x1 >> (N1 - 1 - i)) & 1ull))
+                        continue;
+                    const double *Wi = W + (size_t)i * N0;
+                    for (int j = 0; j < N0; ++j)
+                        if ((x0 >> (N0 - 1 - j)) & 1ull)
+                            E += sgn * Wi[j];
+                }
+                const double Er = bf_round(dtype, E);
+                if (Er > Emin)
+                    continue;
+                if (Er == Emin) {
+                    if (n < maxN) {
+                        xs0[n] = x0;
+                        xs1[n] = x1;
+                        ++n;
+                    }
+                } else {
+                    Emin = Er;
+                    n = 0;
+                    xs0[n] = x0;
+                    xs1[n] = x1;
+                    ++n;
+                }
+            }
+        }
+    }
+    *EminOut = (om == SQO_MAXIMIZE) ? -Emin : Emin; /* calculate_E, :114-123 */
+    return n;
+}
+
 /* ---- bipartite formulas in double ---- */
 void sqo_bg_hamiltonian_f64(int N0, int N1, const double *b0, const double *b1,
                             const double *W, double *h0, double *h1, double *J, double *c) {

@@ -125,6 +125,11 @@ void sqo_bg_energy_q_f64acc(int N0, int N1, int m, const double *h0, const doubl
  * Returns the number of solutions written to xs (packed, bit N-1-pos = x[pos]). */
 int sqo_dg_bf_search(int dtype, int N, const double *W, int om, int tileSize,
                      double *Emin, unsigned long long *xs);
+/* bipartite brute-force search (single-worker order); returns the number of
+ * pairs (<= N0 + N1) written to xs0/xs1, -1 for unsupported sizes */
+int sqo_bg_bf_search(int dtype, int N0, int N1, const double *b0, const double *b1, const double *W, int om,
+                     int tileSize0, int tileSize1, double *EminOut, unsigned long long *xs0,
+                     unsigned long long *xs1);
 
 /* ---- expf ---- */
 /* sum over float bit patterns b in [lo, hi) of splitmix64((b << 32) | bits(expf(b))),

@@ -64,6 +64,22 @@ void withBf(sqaod_rocm_dg_bf_searcher *h, Fn &&fn) {
         fn(*bh->d);
 }
 
+struct BgBfHandle {
+    int dtype;
+    BipartiteGraphBFSearcher<float> *f = nullptr;
+    BipartiteGraphBFSearcher<double> *d = nullptr;
+};
+
+template <class Fn>
+void withBgBf(sqaod_rocm_bg_bf_searcher *h, Fn &&fn) {
+    SQ_THROW_IF(h == nullptr, "null searcher handle");
+    BgBfHandle *bh = reinterpret_cast<BgBfHandle *>(h);
+    if (bh->dtype == 0)
+        fn(*bh->f);
+    else
+        fn(*bh->d);
+}
+
 struct FormulasHandle {
     Formulas<float> f;
     Formulas<double> d;
@@ -750,6 +766,136 @@ int sqaod_rocm_dg_bf_searcher_get_x(sqaod_rocm_dg_bf_searcher *b, int8_t *x, int
 int sqaod_rocm_dg_bf_searcher_get_E(sqaod_rocm_dg_bf_searcher *b, void *E, int n) {
     return guarded([&] {
         withBf(b, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            const int ne = s.numEnergies();
+            SQ_THROW_IF(n != ne, "get_E: expected %d entries, got %d", ne, n);
+            s.get_E(static_cast<real *>(E));
+        });
+    });
+}
+
+/* ---------------- bipartite graph brute-force searcher ---------------- */
+
+int sqaod_rocm_bg_bf_searcher_new(int dtype, sqaod_rocm_bg_bf_searcher **out) {
+    return guarded([&] {
+        SQ_THROW_IF(out == nullptr, "null output pointer");
+        SQ_THROW_IF(dtype != 0 && dtype != 1, "dtype must be 0 (float32) or 1 (float64)");
+        auto *h = new BgBfHandle;
+        h->dtype = dtype;
+        if (dtype == 0)
+            h->f = new BipartiteGraphBFSearcher<float>();
+        else
+            h->d = new BipartiteGraphBFSearcher<double>();
+        *out = reinterpret_cast<sqaod_rocm_bg_bf_searcher *>(h);
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_delete(sqaod_rocm_bg_bf_searcher *b) {
+    return guarded([&] {
+        if (!b)
+            return;
+        BgBfHandle *h = reinterpret_cast<BgBfHandle *>(b);
+        delete h->f;
+        delete h->d;
+        delete h;
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_assign_device(sqaod_rocm_bg_bf_searcher *b, sqaod_rocm_device *dev) {
+    return guarded([&] {
+        SQ_THROW_IF(!dev, "null device");
+        withBgBf(b, [&](auto &s) { s.assignDevice(*dev->dev); });
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_set_qubo(sqaod_rocm_bg_bf_searcher *b, const void *b0, int N0, const void *b1, int N1,
+                                       const void *W, int optimize) {
+    return guarded([&] {
+        SQ_THROW_IF(!b0 || !b1 || !W, "null b0, b1 or W");
+        withBgBf(b, [&](auto &s) {
+            using real = typename std::remove_reference_t<decltype(s)>::real_type;
+            s.setQUBO(static_cast<const real *>(b0), static_cast<const real *>(b1), static_cast<const real *>(W), N0,
+                      N1, optimize ? optMaximize : optMinimize);
+        });
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_get_problem_size(sqaod_rocm_bg_bf_searcher *b, int *N0, int *N1) {
+    return guarded([&] { withBgBf(b, [&](auto &s) { s.getProblemSize(N0, N1); }); });
+}
+
+int sqaod_rocm_bg_bf_searcher_set_preference(sqaod_rocm_bg_bf_searcher *b, const char *name, const char *value) {
+    return guarded([&] {
+        SQ_THROW_IF(!name || !value, "null preference");
+        withBgBf(b, [&](auto &s) { s.setPreference(name, value); });
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_get_preferences(sqaod_rocm_bg_bf_searcher *b, char *buf, size_t buflen,
+                                              size_t *needed) {
+    return guarded([&] {
+        std::string out;
+        withBgBf(b, [&](auto &s) { out = joinPrefs(s.getPreferences()); });
+        copyOut(out, buf, buflen, needed);
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_prepare(sqaod_rocm_bg_bf_searcher *b) {
+    return guarded([&] { withBgBf(b, [&](auto &s) { s.prepare(); }); });
+}
+
+int sqaod_rocm_bg_bf_searcher_calculate_E(sqaod_rocm_bg_bf_searcher *b) {
+    return guarded([&] { withBgBf(b, [&](auto &s) { s.calculate_E(); }); });
+}
+
+int sqaod_rocm_bg_bf_searcher_make_solution(sqaod_rocm_bg_bf_searcher *b) {
+    return guarded([&] { withBgBf(b, [&](auto &s) { s.makeSolution(); }); });
+}
+
+int sqaod_rocm_bg_bf_searcher_search_range(sqaod_rocm_bg_bf_searcher *b, int *done, unsigned long long *cur_x0,
+                                           unsigned long long *cur_x1) {
+    return guarded([&] {
+        withBgBf(b, [&](auto &s) {
+            unsigned long long x0 = 0, x1 = 0;
+            bool d = s.searchRange(&x0, &x1);
+            if (done)
+                *done = d ? 1 : 0;
+            if (cur_x0)
+                *cur_x0 = x0;
+            if (cur_x1)
+                *cur_x1 = x1;
+        });
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_search(sqaod_rocm_bg_bf_searcher *b) {
+    return guarded([&] { withBgBf(b, [&](auto &s) { s.search(); }); });
+}
+
+int sqaod_rocm_bg_bf_searcher_get_n_solutions(sqaod_rocm_bg_bf_searcher *b, int *n) {
+    return guarded([&] {
+        SQ_THROW_IF(!n, "null output pointer");
+        withBgBf(b, [&](auto &s) { *n = s.numSolutions(); });
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_get_x(sqaod_rocm_bg_bf_searcher *b, int8_t *x0, int8_t *x1, int n, int N0, int N1) {
+    return guarded([&] {
+        withBgBf(b, [&](auto &s) {
+            const int ns = s.numSolutions();
+            int P0 = 0, P1 = 0;
+            s.getProblemSize(&P0, &P1);
+            SQ_THROW_IF(n != ns || N0 != P0 || N1 != P1, "get_x: expected %d pairs of (%d, %d), got %d of (%d, %d)",
+                        ns, P0, P1, n, N0, N1);
+            if (ns)
+                s.get_x(x0, x1);
+        });
+    });
+}
+
+int sqaod_rocm_bg_bf_searcher_get_E(sqaod_rocm_bg_bf_searcher *b, void *E, int n) {
+    return guarded([&] {
+        withBgBf(b, [&](auto &s) {
             using real = typename std::remove_reference_t<decltype(s)>::real_type;
             const int ne = s.numEnergies();
             SQ_THROW_IF(n != ne, "get_E: expected %d entries, got %d", ne, n);

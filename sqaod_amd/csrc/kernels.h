@@ -259,6 +259,29 @@ void fp8DecodeTable(float out[256]);
  * wave loads). */
 inline int fp8RowStride(int Np) { return (Np + 1023) / 1024 * 1024; }
 
+/* ---------------- bipartite brute-force search (bipartite_bf.hip) ---------------- */
+
+constexpr int kBGBFHighPerBlock = 16; /* high x0 patterns per block */
+
+/* One scan rectangle: x1 in [x1b, x1b + nX1), high x0 patterns xH in [hb, he)
+ * (x0 = xH << L | xL, all 2^L low patterns); sgn = -1 maximizes. */
+struct BGBFArgs {
+    int N0, N1, L, nH, K;
+    double sgn;
+    unsigned long long x1b, nX1, hb, he;
+};
+BGBFArgs bgBFPlan(int N0, int N1);
+unsigned long long bgBFBlocks(const BGBFArgs &a);
+/* blockMin[bgBFBlocks(a)]: each block's minimum E */
+template <class real>
+void bgBFScan(const BGBFArgs &a, const real *d_b0, const real *d_b1, const real *d_W, real *d_blockMin,
+              hipStream_t stream);
+/* blocks whose minimum is Emin write their first K minimisers (ascending x0,
+ * pairs (x0, x1) as 2 u64) to pairs[block * 2K ...] and the count to counts[block] */
+template <class real>
+void bgBFSelect(const BGBFArgs &a, const real *d_b0, const real *d_b1, const real *d_W, const real *d_blockMin,
+                real Emin, unsigned long long *d_pairs, int *d_counts, hipStream_t stream);
+
 /* ---------------- energies (energy.hip) ---------------- */
 
 /* FP64 recompute E[y] = -c - h.q_y - q_y^T J q_y (J N x Np real, symmetric

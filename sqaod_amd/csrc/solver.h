@@ -383,6 +383,51 @@ private:
 };
 
 /* ------------------------------------------------------------------ */
+/* Bipartite-graph brute-force searcher (CPUBipartiteGraphBFSearcher /   */
+/* CUDABipartiteGraphBFSearcher)                                         */
+/* ------------------------------------------------------------------ */
+template <class real>
+class BipartiteGraphBFSearcher : public SolverState {
+public:
+    using real_type = real;
+    void assignDevice(Device &dev);
+    void setPreference(const std::string &name, const std::string &value);
+    PrefList getPreferences() const;
+    /* W is N1 x N0, N0, N1 <= 63 */
+    void setQUBO(const real *b0, const real *b1, const real *W, int N0, int N1, OptimizeMethod om);
+    void getProblemSize(int *N0, int *N1) const;
+    void prepare();
+    /* scans the next rectangle of pairs; true once every pair is scanned;
+     * *curX0 / *curX1: the position reached (x0 tile start, x1) */
+    bool searchRange(unsigned long long *curX0, unsigned long long *curX1);
+    void search();
+    void makeSolution();
+    void calculate_E();
+    int numSolutions();
+    int numEnergies();
+    void get_x(int8_t *x0, int8_t *x1); /* numSolutions() x N0, x N1 */
+    void get_E(real *E);
+
+private:
+    void ensureDevice();
+    unsigned long long visitKey(unsigned long long x0, unsigned long long x1) const;
+
+    Device *dev_ = nullptr;
+    int N0_ = 0, N1_ = 0;
+    OptimizeMethod om_ = optMinimize;
+    int tileSize0_ = 1024, tileSize1_ = 1024; /* CPUBipartiteGraphBFSearcher.cpp:16-17 */
+    unsigned long long T0_ = 1024;
+    BGBFArgs plan_{};
+    unsigned long long hNext_ = 0, x1Next_ = 0, x1Chunk_ = 0, hChunk_ = 0;
+    real Emin_ = 0;
+    std::vector<std::pair<unsigned long long, unsigned long long>> list_; /* (x0, x1), visiting order */
+    std::vector<real> E_;
+    DevBuf<real> d_b0_, d_b1_, d_W_, d_blockMin_;
+    DevBuf<unsigned long long> d_pairs_;
+    DevBuf<int> d_counts_;
+};
+
+/* ------------------------------------------------------------------ */
 /* Bipartite-graph annealer (CPUBipartiteGraphAnnealer /                 */
 /* CUDABipartiteGraphAnnealer)                                           */
 /* ------------------------------------------------------------------ */

@@ -110,6 +110,22 @@ _SIGS = {
                                                                       _vp, _vp, _i32]),
     "sqaod_rocm_expf_checksum": (_i32, [_u64, _u64, ctypes.POINTER(_u64)]),
     "sqaod_rocm_fp8_decode_table": (_i32, [_vp]),
+    "sqaod_rocm_bg_bf_searcher_new": (_i32, [_i32, _PP]),
+    "sqaod_rocm_bg_bf_searcher_delete": (_i32, [_vp]),
+    "sqaod_rocm_bg_bf_searcher_assign_device": (_i32, [_vp, _vp]),
+    "sqaod_rocm_bg_bf_searcher_set_qubo": (_i32, [_vp, _vp, _i32, _vp, _i32, _vp, _i32]),
+    "sqaod_rocm_bg_bf_searcher_get_problem_size": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaod_rocm_bg_bf_searcher_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "sqaod_rocm_bg_bf_searcher_get_preferences": (_i32, [_vp, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
+    "sqaod_rocm_bg_bf_searcher_prepare": (_i32, [_vp]),
+    "sqaod_rocm_bg_bf_searcher_calculate_E": (_i32, [_vp]),
+    "sqaod_rocm_bg_bf_searcher_make_solution": (_i32, [_vp]),
+    "sqaod_rocm_bg_bf_searcher_search_range": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_u64),
+                                                      ctypes.POINTER(_u64)]),
+    "sqaod_rocm_bg_bf_searcher_search": (_i32, [_vp]),
+    "sqaod_rocm_bg_bf_searcher_get_n_solutions": (_i32, [_vp, ctypes.POINTER(_i32)]),
+    "sqaod_rocm_bg_bf_searcher_get_x": (_i32, [_vp, _vp, _vp, _i32, _i32, _i32]),
+    "sqaod_rocm_bg_bf_searcher_get_E": (_i32, [_vp, _vp, _i32]),
     "sqaod_rocm_dense_batch_new": (_i32, [_PP]),
     "sqaod_rocm_dense_batch_delete": (_i32, [_vp]),
     "sqaod_rocm_dense_batch_set_preference": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
@@ -460,6 +476,89 @@ class _DenseGraphBFSearcher:
     @staticmethod
     def search(cobj, dtype):
         _check(_lib.sqaod_rocm_dg_bf_searcher_search(cobj))
+
+
+class _BipartiteGraphBFSearcher:
+    """rocm_bg_bf_searcher module: bf_searcher.inc:479-494 method table, bipartite form."""
+
+    @staticmethod
+    def new(dtype):
+        out = _vp()
+        _check(_lib.sqaod_rocm_bg_bf_searcher_new(_dt(dtype), ctypes.byref(out)))
+        return out.value
+
+    @staticmethod
+    def delete(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_bf_searcher_delete(cobj))
+
+    @staticmethod
+    def assign_device(cobj, devobj, dtype):
+        _check(_lib.sqaod_rocm_bg_bf_searcher_assign_device(cobj, devobj))
+
+    @staticmethod
+    def set_qubo(cobj, b0, b1, W, optimize, dtype):
+        b0 = np.ascontiguousarray(b0, dtype)
+        b1 = np.ascontiguousarray(b1, dtype)
+        W = np.ascontiguousarray(W, dtype)
+        if W.ndim != 2 or b0.ndim != 1 or b1.ndim != 1 or W.shape != (b1.shape[0], b0.shape[0]):
+            raise RuntimeError("Shape mismatch: W must be N1 x N0 for b0[N0], b1[N1].")
+        _check(_lib.sqaod_rocm_bg_bf_searcher_set_qubo(cobj, _ptr(b0), b0.shape[0], _ptr(b1), b1.shape[0], _ptr(W),
+                                                      int(optimize)))
+
+    @staticmethod
+    def get_problem_size(cobj, dtype):
+        n0, n1 = _i32(), _i32()
+        _check(_lib.sqaod_rocm_bg_bf_searcher_get_problem_size(cobj, ctypes.byref(n0), ctypes.byref(n1)))
+        return n0.value, n1.value
+
+    @staticmethod
+    def set_preferences(cobj, prefs, dtype):
+        _set_prefs(_lib.sqaod_rocm_bg_bf_searcher_set_preference, cobj, prefs)
+
+    @staticmethod
+    def get_preferences(cobj, dtype):
+        return _get_prefs(_lib.sqaod_rocm_bg_bf_searcher_get_preferences, cobj)
+
+    @staticmethod
+    def get_x(cobj, dtype):
+        n = _i32()
+        _check(_lib.sqaod_rocm_bg_bf_searcher_get_n_solutions(cobj, ctypes.byref(n)))
+        N0, N1 = _BipartiteGraphBFSearcher.get_problem_size(cobj, dtype)
+        x0 = np.empty((n.value, N0), np.int8)
+        x1 = np.empty((n.value, N1), np.int8)
+        _check(_lib.sqaod_rocm_bg_bf_searcher_get_x(cobj, _ptr(x0), _ptr(x1), n.value, N0, N1))
+        return [(x0[k].copy(), x1[k].copy()) for k in range(n.value)]
+
+    @staticmethod
+    def get_E(cobj, dtype):
+        n = _i32()
+        _check(_lib.sqaod_rocm_bg_bf_searcher_get_n_solutions(cobj, ctypes.byref(n)))
+        E = np.empty(max(1, n.value), dtype)
+        _check(_lib.sqaod_rocm_bg_bf_searcher_get_E(cobj, _ptr(E), E.shape[0]))
+        return E
+
+    @staticmethod
+    def prepare(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_bf_searcher_prepare(cobj))
+
+    @staticmethod
+    def calculate_E(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_bf_searcher_calculate_E(cobj))
+
+    @staticmethod
+    def make_solution(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_bf_searcher_make_solution(cobj))
+
+    @staticmethod
+    def search_range(cobj, dtype):
+        done, c0, c1 = _i32(), _u64(), _u64()
+        _check(_lib.sqaod_rocm_bg_bf_searcher_search_range(cobj, ctypes.byref(done), ctypes.byref(c0),
+                                                          ctypes.byref(c1)))
+        return bool(done.value), c0.value, c1.value
+
+    @staticmethod
+    def search(cobj, dtype):
+        _check(_lib.sqaod_rocm_bg_bf_searcher_search(cobj))
 
 
 class _BipartiteGraphAnnealer:
@@ -857,6 +956,7 @@ rocm_device = _Device
 rocm_dg_annealer = _DenseGraphAnnealer
 rocm_dg_bf_searcher = _DenseGraphBFSearcher
 rocm_bg_annealer = _BipartiteGraphAnnealer
+rocm_bg_bf_searcher = _BipartiteGraphBFSearcher
 rocm_formulas = _Formulas
 rocm_dense_batch = _DenseBatch
 rocm_server = _Server

@@ -607,3 +607,57 @@ def test_dense_stationary_distribution_is_boltzmann(oracle, N, m, G):
         logw = -beta * (E.mean(axis=1) + 0.5 * coef * dot)
     counts = _state_histogram(an, lambda: an.get_q().reshape(-1).astype(np.int8), 40000, G, beta)
     assert _total_variation(counts, states, logw) < 0.08
+
+
+# ---------------- bipartite brute-force search ----------------
+
+def _bg_all_pairs(b0, b1, W):
+    """E[x0, x1] for every packed pair (bit N-1-pos = x[pos]), exact."""
+    N0, N1 = len(b0), len(b1)
+    X0 = ((np.arange(1 << N0)[:, None] >> np.arange(N0 - 1, -1, -1)[None, :]) & 1).astype(np.float64)
+    X1 = ((np.arange(1 << N1)[:, None] >> np.arange(N1 - 1, -1, -1)[None, :]) & 1).astype(np.float64)
+    return (X0 @ b0)[:, None] + (X1 @ b1)[None, :] + X0 @ W.T @ X1.T
+
+
+@pytest.mark.parametrize("opt,sign,Eexp,bit", [(0, 1, 0.0, 0), (0, -1, -(64 + 16), 1), (1, 1, 64 + 16, 1),
+                                              (1, -1, 0.0, 0)])
+def test_bg_bf_known_answers(oracle, opt, sign, Eexp, bit):
+    """test_bipartite_graph_bf_searcher.py:62-110: +-ones problems at 8 x 8."""
+    N0 = N1 = 8
+    E, xs0, xs1 = oracle.bipartite_bf_search(sign * np.ones(N0), sign * np.ones(N1), sign * np.ones((N1, N0)),
+                                             optimize=opt)
+    assert E == Eexp and len(xs0) == 1
+    full = (1 << 8) - 1
+    assert xs0[0] == (full if bit else 0) and xs1[0] == (full if bit else 0)
+
+
+@pytest.mark.parametrize("N0,N1,T0,seed", [(5, 4, 1024, 1), (3, 7, 4, 2), (6, 6, 16, 3), (7, 2, 32, 4)])
+def test_bg_bf_matches_brute_force_in_visiting_order(oracle, N0, N1, T0, seed):
+    """Integer b0, b1, W in [-2, 2] (many ties): Emin equals the exhaustive
+    minimum, and the list is the first N0 + N1 minimisers in the one-worker
+    order (x0 tile, x1, x0 within the tile) -- computed here by sorting keys."""
+    rng = np.random.default_rng(seed)
+    b0 = rng.integers(-2, 3, N0).astype(np.float64)
+    b1 = rng.integers(-2, 3, N1).astype(np.float64)
+    W = rng.integers(-2, 3, (N1, N0)).astype(np.float64)
+    for opt, sgn in ((0, 1.0), (1, -1.0)):
+        E = sgn * _bg_all_pairs(b0, b1, W)
+        Emin = E.min()
+        x0s, x1s = np.nonzero(E == Emin)
+        T = min(T0, 1 << N0)
+        key = (x0s // T) * (1 << N1) * T + x1s * T + x0s % T
+        order = np.argsort(key)[:N0 + N1]
+        Eo, xs0, xs1 = oracle.bipartite_bf_search(b0, b1, W, optimize=opt, tile_size0=T0)
+        assert Eo == sgn * Emin
+        assert np.array_equal(xs0, x0s[order].astype(np.uint64)) and np.array_equal(xs1, x1s[order].astype(np.uint64))
+
+
+def test_bg_bf_tie_list_capped(oracle):
+    """All 2^(N0+N1) pairs tie at E = 0: the list holds the first N0 + N1 pairs:
+    x0 = 0..T0-1 of tile 0 under x1 = 0, then x1 = 1 ..."""
+    N0, N1 = 3, 4
+    E, xs0, xs1 = oracle.bipartite_bf_search(np.zeros(N0), np.zeros(N1), np.zeros((N1, N0)))
+    assert E == 0.0 and len(xs0) == N0 + N1
+    assert list(xs0) == [0, 1, 2, 3, 4, 5, 6] and list(xs1) == [0] * 7
+    E, xs0, xs1 = oracle.bipartite_bf_search(np.zeros(N0), np.zeros(N1), np.zeros((N1, N0)), tile_size0=2)
+    assert list(xs0) == [0, 1, 0, 1, 0, 1, 0] and list(xs1) == [0, 0, 1, 1, 2, 2, 3]
