@@ -120,28 +120,45 @@ __device__ __forceinline__ void refreshBF16x2Block(const uint4 *frag, const uint
                 bh[buf][d] = p[0];
                 bl[buf][d] = p[kWave];
             }
-            /* words kb0 .. kb0+3 of row y0 + 64 r + lane (kb0 is a multiple of 4, Nw of 8) */
+            /* words kb0 .. kb0+3 of row y0 + 64 r + lane (kb0 is a multiple of 4, Nw of 8);
+             * rows past m read row m - 1 and are zeroed, so every load is
+             * unconditional and the waits before compute count precisely */
 #pragma unroll
             for (int r = 0; r < kTiles / 4; ++r) {
                 const int y = y0 + 64 * r + lane;
-                qw[buf][r] = (y < m) ? *reinterpret_cast<const uint4 *>(Qbits + (size_t)y * Nw + kb0)
-                                     : uint4{0u, 0u, 0u, 0u};
+                const uint4 v = *reinterpret_cast<const uint4 *>(Qbits + (size_t)(y < m ? y : m - 1) * Nw + kb0);
+                qw[buf][r] = (y < m) ? v : uint4{0u, 0u, 0u, 0u};
             }
         };
+        /* A operands of four tiles are gathered (lane shuffle, then table
+         * read) before their MFMAs, so the LDS round trips of a group overlap;
+         * a pass with all 16 tiles in range has no per-tile branch at all */
+        const bool fullPass = y0 + kRowsPerPass <= m;
         auto compute = [&](int buf) {
 #pragma unroll
             for (int d = 0; d < kDepth; ++d) {
                 const bf16x8 bhi = __builtin_bit_cast(bf16x8, bh[buf][d]);
                 const bf16x8 blo = __builtin_bit_cast(bf16x8, bl[buf][d]);
 #pragma unroll
-                for (int t = 0; t < kTiles; ++t) {
-                    if (y0 + 16 * t < m) { /* wave-uniform */
-                        const uint4 q4 = qw[buf][t >> 2];
-                        const uint32_t qd = d == 0 ? q4.x : d == 1 ? q4.y : d == 2 ? q4.z : q4.w;
-                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * (t & 3) + c, kWave);
-                        const bf16x8 a = __builtin_bit_cast(bf16x8, lut[(word >> (8 * g)) & 0xffu]);
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bhi, acc[t], 0, 0, 0);
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, blo, acc[t], 0, 0, 0);
+                for (int t0 = 0; t0 < kTiles; t0 += 4) {
+                    if (!fullPass && y0 + 16 * t0 >= m) /* wave-uniform */
+                        break;
+                    const uint4 q4 = qw[buf][t0 >> 2];
+                    const uint32_t qd = d == 0 ? q4.x : d == 1 ? q4.y : d == 2 ? q4.z : q4.w;
+                    uint4 a4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * u + c, kWave);
+                        a4[u] = lut[(word >> (8 * g)) & 0xffu];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int t = t0 + u;
+                        if (fullPass || y0 + 16 * t < m) {
+                            const bf16x8 a = __builtin_bit_cast(bf16x8, a4[u]);
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bhi, acc[t], 0, 0, 0);
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, blo, acc[t], 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -211,26 +228,37 @@ __device__ __forceinline__ void refreshF32Block(const float4 *frag, const uint32
 #pragma unroll
             for (int r = 0; r < kTiles / 4; ++r) {
                 const int y = y0 + 64 * r + lane;
-                qw[buf][r] = (y < m) ? *reinterpret_cast<const uint2 *>(Qbits + (size_t)y * Nw + kb0 / 2)
-                                     : uint2{0u, 0u};
+                const uint2 v = *reinterpret_cast<const uint2 *>(Qbits + (size_t)(y < m ? y : m - 1) * Nw + kb0 / 2);
+                qw[buf][r] = (y < m) ? v : uint2{0u, 0u};
             }
         };
+        const bool fullPass = y0 + kRowsPerPass <= m;
         auto compute = [&](int buf) {
 #pragma unroll
             for (int d = 0; d < kDepth; ++d) {
                 const float b[4] = {bv[buf][d].x, bv[buf][d].y, bv[buf][d].z, bv[buf][d].w};
 #pragma unroll
-                for (int t = 0; t < kTiles; ++t) {
-                    if (y0 + 16 * t < m) {
-                        const uint2 q2 = qw[buf][t >> 2];
-                        const uint32_t qd = (d >> 1) ? q2.y : q2.x;
-                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * (t & 3) + c, kWave);
-                        /* lane group g holds k = 16 kb + 4 g + s for step s (same for A and B) */
-                        const uint32_t nib = word >> (16 * (d & 1) + 4 * g);
+                for (int t0 = 0; t0 < kTiles; t0 += 4) {
+                    if (!fullPass && y0 + 16 * t0 >= m) /* wave-uniform */
+                        break;
+                    const uint2 q2 = qw[buf][t0 >> 2];
+                    const uint32_t qd = (d >> 1) ? q2.y : q2.x;
+                    uint32_t nib[4];
 #pragma unroll
-                        for (int s = 0; s < 4; ++s) {
-                            const float a = ((nib >> s) & 1u) ? 1.f : -1.f;
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc[t], 0, 0, 0);
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * u + c, kWave);
+                        /* lane group g holds k = 16 kb + 4 g + s for step s (same for A and B) */
+                        nib[u] = word >> (16 * (d & 1) + 4 * g);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int t = t0 + u;
+                        if (fullPass || y0 + 16 * t < m) {
+#pragma unroll
+                            for (int s = 0; s < 4; ++s) {
+                                const float a = ((nib[u] >> s) & 1u) ? 1.f : -1.f;
+                                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc[t], 0, 0, 0);
+                            }
                         }
                     }
                 }
@@ -313,9 +341,8 @@ __device__ __forceinline__ void refreshFP8Block(const uint4 *frag, const float *
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
                 const int y = y0 + 16 * t + c;
-                qv[buf][t] = (y0 + 16 * t < m && y < m)
-                                 ? *reinterpret_cast<const uint2 *>(Qbits + (size_t)y * Nw + 8 * s + 2 * g)
-                                 : uint2{0u, 0u};
+                const uint2 v = *reinterpret_cast<const uint2 *>(Qbits + (size_t)(y < m ? y : m - 1) * Nw + 8 * s + 2 * g);
+                qv[buf][t] = (y < m) ? v : uint2{0u, 0u};
             }
         };
         auto compute = [&](int buf) {
