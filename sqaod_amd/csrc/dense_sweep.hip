@@ -221,18 +221,34 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
      * float the fp32 form adds for the same J'. */
     bool rowPending = false;
     real rowD = 0;
+    /* four 256-element slices per round: all their LDS reads are issued
+     * before the first write, so the round trips overlap (Np is a multiple
+     * of 256; the tail round covers what is left) */
     auto applyPendingRow = [&]() {
-        for (int j = lane * 4; j < Np; j += kWave * 4) {
-            real fv[4], sv[4];
-            ldsLoad4(F + j, fv);
-            if constexpr (FP8)
-                fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(stage8 + j), sv);
-            else
-                ldsLoad4(stage + j, sv);
+        constexpr int kR = 4;
+        for (int j0 = lane * 4; j0 < Np; j0 += kR * kWave * 4) {
+            real fv[kR][4], sv[kR][4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-                fv[t] = fv[t] + rowD * sv[t];
-            ldsStore4(F + j, fv);
+            for (int r = 0; r < kR; ++r) {
+                const int j = j0 + r * kWave * 4;
+                if (j < Np) {
+                    ldsLoad4(F + j, fv[r]);
+                    if constexpr (FP8)
+                        fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(stage8 + j), sv[r]);
+                    else
+                        ldsLoad4(stage + j, sv[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                const int j = j0 + r * kWave * 4;
+                if (j < Np) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        fv[r][t] = fv[r][t] + rowD * sv[r][t];
+                    ldsStore4(F + j, fv[r]);
+                }
+            }
         }
         rowPending = false;
     };
