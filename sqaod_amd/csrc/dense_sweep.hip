@@ -71,6 +71,82 @@ __device__ __forceinline__ int firstLane(unsigned long long mask) {
     return mask ? (int)__builtin_ctzll(mask) : kWave;
 }
 
+/* LDS DMA issued from inline assembly: lane i's 4 (16) bytes from src land at
+ * ldsDst + 4i (16i).  The compiler sees neither the op nor its LDS write, so
+ * it adds no memory waits for it (for the builtin form it waits for every
+ * outstanding DMA before any later LDS access, which serialises the batch
+ * loop); every read of a DMA destination follows an explicit vmWait instead.
+ * Ops the compiler does not count only make its own waits stricter (vmcnt
+ * retires in issue order), never weaker.  M0 carries the LDS address; nothing
+ * else in the sweep kernels uses M0.  kCoherent: the cache policy of a relaxed
+ * atomic load at agent scope (decisions published by other waves). */
+/* the M0 clobber below is deliberate (clang warns about any M0 clobber) */
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ uint32_t ldsAddress(const void *p) {
+    return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p);
+}
+template <bool kCoherent>
+__device__ __forceinline__ void dmaDword(const void *src, uint32_t lds) {
+    const uint32_t m = __builtin_amdgcn_readfirstlane(lds);
+    if constexpr (kCoherent)
+        asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off sc1" ::"v"(src), "s"(m) : "memory", "m0");
+    else
+        asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(src), "s"(m) : "memory", "m0");
+}
+__device__ __forceinline__ void dmaDwordx4(const void *src, uint32_t lds) {
+    const uint32_t m = __builtin_amdgcn_readfirstlane(lds);
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(m) : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+/* wait until at most C vector-memory ops are outstanding */
+template <int C>
+__device__ __forceinline__ void vmWaitImm() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(C) : "memory");
+}
+template <int LO, int HI>
+__device__ __forceinline__ void vmWaitRange(int c) {
+    if constexpr (LO == HI) {
+        vmWaitImm<LO>();
+    } else {
+        constexpr int MID = (LO + HI + 1) / 2;
+        if (c >= MID)
+            vmWaitRange<MID, HI>(c);
+        else
+            vmWaitRange<LO, MID - 1>(c);
+    }
+}
+/* vmcnt(c) for a wave-uniform c (counts above 63 wait for 63) */
+__device__ __forceinline__ void vmWait(int c) {
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (c == 1) /* the common case: only a decision store since */
+        vmWaitImm<1>();
+    else
+        vmWaitRange<0, 63>(c < 63 ? c : 63);
+}
+/* this wave's LDS reads have returned (before a DMA overwrites what they read) */
+__device__ __forceinline__ void ldsWait() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+/* Copy kEntries entries of ES bytes, from entry e0 of src, into LDS at lds by
+ * DMA, skipping entries >= limit (their LDS words keep old contents). */
+template <int ES, bool kCoherent, int kEntries>
+__device__ __forceinline__ void dmaEntries(const void *src, uint32_t lds, int e0, int limit) {
+    const int lane = threadIdx.x & (kWave - 1);
+    constexpr int kOps = kEntries * ES / (4 * kWave);
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src) + lane;
+    if (e0 + kEntries <= limit) {
+#pragma unroll
+        for (int j = 0; j < kOps; ++j)
+            dmaDword<kCoherent>(s + j * kWave, lds + j * kWave * 4);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kOps; ++j)
+            if (e0 + (j * kWave + lane) * 4 / ES < limit)
+                dmaDword<kCoherent>(s + j * kWave, lds + j * kWave * 4);
+    }
+}
+
 /* entries of the per-slice pending-site table: a power of two >= min(Np, 4096);
  * sites are hashed by their low bits, a collision only delays a lane. */
 __host__ __device__ inline int pendingTableSize(int Np) {
@@ -153,8 +229,18 @@ __device__ __forceinline__ void fp8x4ToFloat(uint32_t w, float v[4]) {
     v[0] = lo.x; v[1] = lo.y; v[2] = hi.x; v[3] = hi.y;
 }
 
-/* Bytes of LDS one wave's slice needs (fields, row stage, pending table,
- * spin bits and the two mirrors), for a row length Np. */
+/* Fixed part of a wave's LDS region: the two neighbour windows (decisions and
+ * sites of kWin entries each), the own-stream ring (sites and uniforms of
+ * kRing entries) and the fp8 row scale (a 16-byte slot). */
+constexpr int kWin = 2 * kWave;
+constexpr int kRing = 4 * kWave;
+template <class real>
+__host__ __device__ constexpr size_t sweepFixedBytes() {
+    return (size_t)4 * kWin * 4 + (size_t)kRing * (4 + sizeof(real)) + 16;
+}
+
+/* Bytes of LDS one wave's slice needs (the fixed part, fields, row stage,
+ * pending table, spin bits and the two mirrors), for a row length Np. */
 template <class real, int MODE>
 __host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8) {
     constexpr bool PLAIN = MODE == kModePlain;
@@ -162,7 +248,7 @@ __host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8) {
     constexpr bool kDefer = !PLAIN && sizeof(real) == 4;
     const size_t stageBytes = kDefer ? (FP8 ? (size_t)ld8 : (size_t)Np * sizeof(real)) : 0;
     const size_t fieldBytes = PLAIN ? 0 : (size_t)Np * sizeof(real) + stageBytes;
-    return fieldBytes + (size_t)pendingTableSize(Np) * 4 + (size_t)3 * (Np / 32) * 4;
+    return sweepFixedBytes<real>() + fieldBytes + (size_t)pendingTableSize(Np) * 4 + (size_t)3 * (Np / 32) * 4;
 }
 
 /* One wave runs the whole sweep of slice y: N attempts, with its LDS region
@@ -181,12 +267,24 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     constexpr bool kDefer = !PLAIN && sizeof(real) == 4; /* deferred row updates, staged in LDS */
     const size_t stageBytes = kDefer ? (FP8 ? (size_t)a.ld8 : (size_t)Np * sizeof(real)) : 0;
     const size_t fieldBytes = PLAIN ? 0 : (size_t)Np * sizeof(real) + stageBytes;
-    real *F = reinterpret_cast<real *>(p);
+    uint32_t *const winD = reinterpret_cast<uint32_t *>(p); /* [nb][kWin] neighbour decisions */
+    uint32_t *const winS = winD + 2 * kWin;                  /* [nb][kWin] neighbour sites */
+    uint32_t *const ringX = winS + 2 * kWin;                 /* own sites, entry e at e % kRing */
+    real *const ringU = reinterpret_cast<real *>(ringX + kRing);
+    float *const scaleSlot = reinterpret_cast<float *>(ringU + kRing); /* FP8: the last flip's row scale */
+    unsigned char *const body = p + sweepFixedBytes<real>();
+    real *F = reinterpret_cast<real *>(body);
     real *stage = F + Np; /* kDefer: the J row of the last flip, as loaded */
     const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(stage); /* FP8: the e4m3 row */
-    int *tbl = reinterpret_cast<int *>(p + fieldBytes);
+    int *tbl = reinterpret_cast<int *>(body + fieldBytes);
     uint32_t *Qb = reinterpret_cast<uint32_t *>(tbl + tblN);
     uint32_t *Mir[2] = {Qb + Nw, Qb + 2 * Nw};
+    /* LDS byte addresses of the DMA destinations (wave-uniform) */
+    const uint32_t winDL = ldsAddress(p), winSL = winDL + 2 * kWin * 4, ringXL = winSL + 2 * kWin * 4;
+    const uint32_t ringUL = ringXL + kRing * 4, scaleL = ringUL + kRing * (uint32_t)sizeof(real);
+    const uint32_t stageL = winDL + (uint32_t)(sweepFixedBytes<real>() + (size_t)Np * sizeof(real));
+    (void)scaleL;
+    (void)stageL;
 
     const long long yN = (long long)y * N;
     const int n0 = (y == 0) ? m - 1 : y - 1;
@@ -253,64 +351,80 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         rowPending = false;
     };
 
-    /* Own attempt stream: a rolling window of 128 entries in two halves
-     * (entries wk + lane and wk + 64 + lane); a batch at k < wk + 64 reads
-     * lanes k - wk .. k - wk + 63 of it, and a half is refilled as soon as it
-     * is consumed, so the stream is rarely on the critical path. */
-    uint32_t wS0, wS1;
-    real wU0, wU1;
-    int wk = 0;
-    auto loadHalf = [&](int base, uint32_t &sv, real &uv) {
-        const int kk = base + lane;
-        sv = kk < N ? a.X[yN + kk] : 0u;
-        uv = kk < N ? a.U[yN + kk] : (real)0;
-    };
-    loadHalf(0, wS0, wU0);
-    loadHalf(kWave, wS1, wU1);
+    /* Memory traffic of the batch loop.  What the loop reads from global
+     * memory arrives in LDS by DMA (dmaDword), so its memory waits are these,
+     * in issue order:
+     *   neighbour phase: the prefetch group -- both neighbour windows for the
+     *     next batch and at most one new quarter of the own-stream ring;
+     *   commit: the decision store (1 op), and after a flip the row and, for
+     *     fp8, its scale (rowOps ops).
+     * A batch waits for the last prefetch group with vmcnt(ops issued since),
+     * which leaves the decision store and the row in flight; the row is
+     * waited for where it is first read (rowCorr). */
+    const int rowOps = kDefer ? (FP8 ? a.ld8 / (kWave * 16) + 1 : Np * (int)sizeof(real) / (kWave * 16)) : 0;
+    int afterGroup = 0; /* counted ops issued since the last prefetch group */
+    real rowSign = 0;   /* FP8: -+2 of the staged row; rowD = rowSign * its scale */
+    (void)rowSign;
 
-    /* Neighbour windows: decisions and sites of entries [applied, applied+128),
-     * fetched one batch ahead. */
-    uint32_t nd[2][2], ns[2][2];
+    /* Own attempt stream: a ring of kRing entries (entry e at e % kRing)
+     * filled a quarter (64 entries) at a time, up to two quarters beyond the
+     * one holding k; a batch reads entries k .. k + 63. */
+    int ringNext = 0; /* quarters [0, ringNext) have been requested */
+    auto requestQuarter = [&]() {
+        const int e0 = ringNext * kWave;
+        const int slot = e0 & (kRing - 1);
+        dmaEntries<4, false, kWave>(a.X + yN + e0, ringXL + slot * 4, e0, N);
+        dmaEntries<(int)sizeof(real), false, kWave>(a.U + yN + e0, ringUL + slot * (uint32_t)sizeof(real), e0, N);
+        ++ringNext;
+    };
+
+    /* Neighbour windows: decisions and sites of entries [applied, applied+128)
+     * in LDS, requested one batch ahead. */
+    uint32_t ns[2][2];
     int nbase[2] = {0, 0};
     auto loadNeighbour = [&](int nb) {
         const long long nN = (long long)nbr[nb] * N;
         const int base = applied[nb];
-        const int e0 = base + lane, e1 = base + kWave + lane;
         nbase[nb] = base;
-        nd[nb][0] = e0 < N ? loadShared32(a.D + nN + e0) : 0u;
-        nd[nb][1] = e1 < N ? loadShared32(a.D + nN + e1) : 0u;
-        ns[nb][0] = e0 < N ? a.X[nN + e0] : 0u;
-        ns[nb][1] = e1 < N ? a.X[nN + e1] : 0u;
+        dmaEntries<4, true, kWin>(a.D + nN + base, winDL + nb * kWin * 4, base, N);
+        dmaEntries<4, false, kWin>(a.X + nN + base, winSL + nb * kWin * 4, base, N);
     };
-    if (!a.sa) {
-        loadNeighbour(0);
-        loadNeighbour(1);
-    }
+    auto prefetch = [&]() {
+        ldsWait(); /* this batch's reads of the windows and the ring have returned */
+        if (!a.sa) {
+            loadNeighbour(0);
+            loadNeighbour(1);
+        }
+        while (ringNext < (k >> 6) + 3 && ringNext * kWave < N)
+            requestQuarter();
+        afterGroup = 0;
+    };
+    prefetch();
+    vmWaitImm<0>();
 
     unsigned long long cyc[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* load wait, neighbours, evaluate, commit, row, blocked */
     (void)cyc;
     while (k < N) {
         SQ_CYC(c0);
         SQ_CYC_WAITALL();
+        vmWait(afterGroup); /* the last prefetch group has landed */
         SQ_CYC(c1);
         SQ_CYC_ADD(cyc[0], c0, c1);
-        if (k >= wk + kWave) {
-            wS0 = wS1;
-            wU0 = wU1;
-            wk += kWave;
-            loadHalf(wk + kWave, wS1, wU1);
-        }
         int limit = N - k;
-        uint32_t s;
-        real u;
-        {
-            const int idx = k - wk + lane; /* < 128 */
-            const uint32_t sa0 = __shfl(wS0, idx & (kWave - 1)), sa1 = __shfl(wS1, idx & (kWave - 1));
-            const real ua0 = __shfl(wU0, idx & (kWave - 1)), ua1 = __shfl(wU1, idx & (kWave - 1));
-            s = idx < kWave ? sa0 : sa1;
-            u = idx < kWave ? ua0 : ua1;
-            if (k + lane >= N)
-                s = 0u;
+        uint32_t s = ringX[(k + lane) & (kRing - 1)];
+        const real u = ringU[(k + lane) & (kRing - 1)];
+        if (k + lane >= N)
+            s = 0u;
+        uint32_t nd[2][2]; /* the windows, read together */
+        auto readWindow = [&](int nb) {
+            nd[nb][0] = winD[nb * kWin + lane];
+            nd[nb][1] = winD[nb * kWin + kWave + lane];
+            ns[nb][0] = winS[nb * kWin + lane];
+            ns[nb][1] = winS[nb * kWin + kWave + lane];
+        };
+        if (!a.sa) {
+            readWindow(0);
+            readWindow(1);
         }
 
         /* neighbours: replay decided prefixes, register pending entries */
@@ -324,15 +438,16 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 for (;;) {
                     const int base = nbase[nb];
                     const int e0 = base + lane, e1 = base + kWave + lane;
-                    const bool dec0 = (nd[nb][0] >> 1) == tag, dec1 = (nd[nb][1] >> 1) == tag;
+                    const uint32_t d0 = nd[nb][0], d1 = nd[nb][1];
+                    const bool dec0 = (d0 >> 1) == tag, dec1 = (d1 >> 1) == tag;
                     const unsigned long long u0 = __ballot(e0 < N && !dec0);
                     const unsigned long long u1 = __ballot(e1 < N && !dec1);
                     const int winLen = N - base < 2 * kWave ? N - base : 2 * kWave;
                     const int pfx = u0 ? firstLane(u0) : (u1 ? kWave + firstLane(u1) : winLen);
                     const int from = applied[nb];
                     const int applyEnd = base + pfx < Kk ? base + pfx : Kk;
-                    const bool x0 = e0 >= from && e0 < applyEnd && (nd[nb][0] & 1u);
-                    const bool x1 = e1 >= from && e1 < applyEnd && (nd[nb][1] & 1u);
+                    const bool x0 = e0 >= from && e0 < applyEnd && (d0 & 1u);
+                    const bool x1 = e1 >= from && e1 < applyEnd && (d1 & 1u);
                     if (__ballot(x0 || x1)) {
                         if (x0)
                             atomicXor(&Mir[nb][ns[nb][0] >> 5], 1u << (ns[nb][0] & 31));
@@ -343,7 +458,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                         applied[nb] = applyEnd;
                     if (pfx == 2 * kWave && applyEnd == base + 2 * kWave && applyEnd < Kk) {
                         /* whole window replayed and still behind: fetch the next one */
+                        ldsWait();
                         loadNeighbour(nb);
+                        vmWaitImm<0>();
+                        readWindow(nb);
                         (void)nN;
                         continue;
                     }
@@ -390,9 +508,9 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         real rowCorr = 0;
         if constexpr (kDefer) {
             if (rowPending) {
-                __builtin_amdgcn_s_waitcnt(0);
-                waveSync();
+                vmWaitImm<0>();
                 if constexpr (FP8) {
+                    rowD = rowSign * scaleSlot[0];
                     float v4[4];
                     fp8x4ToFloat(stage8[s], v4);
                     rowCorr = v4[0];
@@ -402,13 +520,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             }
         }
         /* the next batch's neighbour windows start where this batch's replay
-         * ended: fetch them now, so the loads overlap evaluation and commit */
+         * ended: request them now, so the DMA overlaps evaluation and commit */
         bool pendSetNow[2][2] = {{pendSet[0][0], pendSet[0][1]}, {pendSet[1][0], pendSet[1][1]}};
         uint32_t pendSite[2][2] = {{ns[0][0], ns[0][1]}, {ns[1][0], ns[1][1]}};
-        if (!a.sa) {
-            loadNeighbour(0);
-            loadNeighbour(1);
-        }
+        prefetch();
         SQ_CYC(c2);
         SQ_CYC_ADD(cyc[1], c1, c2);
 
@@ -579,10 +694,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 aborted = true;
                 break;
             }
-            if (!a.sa) {
-                loadNeighbour(0);
-                loadNeighbour(1);
-            }
+            prefetch();
             waveSync();
             {
                 SQ_CYC(cb);
@@ -594,8 +706,9 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         ++nBatches;
 
         /* publish the committed decisions */
-        if (lane < c)
+        if (lane < c) /* lane 0 always stores: one op */
             storeShared32(a.D + yN + k + lane, (tag << 1) | (lane == flipLane ? 1u : 0u));
+        afterGroup += 1;
 
         SQ_CYC(c4);
         if constexpr (kDefer)
@@ -609,25 +722,27 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 atomicXor(&Qb[x >> 5], 1u << (x & 31));
             if constexpr (FP8) {
                 /* the same deferred update from the e4m3 row (ld8 bytes, a
-                 * multiple of 1 KiB) and its scale */
-                rowD = (qx ? -2.f : 2.f) * a.rowScale[x];
+                 * multiple of 1 KiB) and its scale, rowD = -+2 * scale */
+                rowSign = qx ? -2.f : 2.f;
                 const uint8_t *row = a.J8 + (long long)x * a.ld8;
                 waveSync();
+                ldsWait(); /* this wave's reads of `stage` and the scale are done */
                 for (int j0 = 0; j0 < a.ld8; j0 += kWave * 16)
-                    __builtin_amdgcn_global_load_lds((const void *)(row + j0 + lane * 16),
-                                                     (__attribute__((address_space(3))) void *)(stage + j0 / 4), 16,
-                                                     0, 0);
+                    dmaDwordx4(row + j0 + lane * 16, stageL + j0);
+                if (lane == 0)
+                    dmaDword<false>(a.rowScale + x, scaleL);
+                afterGroup += rowOps;
                 rowPending = true;
             } else if constexpr (kDefer) {
                 /* F[y][:] += -2 q_x J[x][:] (J symmetric), deferred: the row
-                 * streams into `stage`, 1 KiB (16 B per lane) per load */
+                 * streams into `stage`, 1 KiB (16 B per lane) per op */
                 rowD = qx ? (real)-2. : (real)2.;
                 const real *row = a.J + (long long)x * Np;
-                waveSync(); /* this wave's reads of `stage` are done */
+                waveSync();
+                ldsWait(); /* this wave's reads of `stage` are done */
                 for (int j0 = 0; j0 < Np; j0 += kWave * 4)
-                    __builtin_amdgcn_global_load_lds((const void *)(row + j0 + lane * 4),
-                                                     (__attribute__((address_space(3))) void *)(stage + j0), 16, 0,
-                                                     0);
+                    dmaDwordx4(row + j0 + lane * 4, stageL + j0 * 4);
+                afterGroup += rowOps;
                 rowPending = true;
             } else if constexpr (!PLAIN) {
                 /* F[y][:] += -2 q_x J[:][x] = -2 q_x J[x][:]  (J symmetric);
@@ -665,10 +780,12 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         SQ_CYC_ADD(cyc[4], c4, c5);
     }
 
+    /* nothing may land in this region after the wave leaves it */
+    vmWaitImm<0>();
     if constexpr (kDefer) {
         if (rowPending && !aborted) {
-            __builtin_amdgcn_s_waitcnt(0);
-            waveSync();
+            if constexpr (FP8)
+                rowD = rowSign * scaleSlot[0];
             applyPendingRow();
             waveSync();
         }
