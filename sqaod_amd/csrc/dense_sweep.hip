@@ -43,7 +43,7 @@ constexpr int kMaxSlicesPerBlock = 4; /* one wave per SIMD, so a wave may hold u
 constexpr size_t kExpTabBytes = 32 * sizeof(uint64_t); /* LDS copy of the exp table */
 
 /* Optional per-phase cycle counters (build with -DSQAOD_SWEEP_CYCLES): shader
- * clock cycles spent per phase, summed into SweepArgs::counters[3..10]. */
+ * clock cycles spent per phase, summed into SweepArgs::counters[3..13]. */
 #ifdef SQAOD_SWEEP_CYCLES
 #define SQ_CYC(v) const unsigned long long v = __builtin_amdgcn_s_memtime()
 #define SQ_CYC_ADD(acc, a, b) (acc) += (b) - (a)
@@ -70,6 +70,11 @@ __device__ __forceinline__ void waveSync() {
 __device__ __forceinline__ int firstLane(unsigned long long mask) {
     return mask ? (int)__builtin_ctzll(mask) : kWave;
 }
+
+/* A wave-uniform int, marked so: control values the compiler cannot prove
+ * uniform (they pass through loops with ballot-dependent exits) would
+ * otherwise live in VGPRs and turn every branch on them into exec masking. */
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 /* LDS DMA issued from inline assembly: lane i's 4 (16) bytes from src land at
  * ldsDst + 4i (16i).  The compiler sees neither the op nor its LDS write, so
@@ -384,7 +389,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     int nbase[2] = {0, 0};
     auto loadNeighbour = [&](int nb) {
         const long long nN = (long long)nbr[nb] * N;
-        const int base = applied[nb];
+        const int base = uni(applied[nb]);
         nbase[nb] = base;
         dmaEntries<4, true, kWin>(a.D + nN + base, winDL + nb * kWin * 4, base, N);
         dmaEntries<4, false, kWin>(a.X + nN + base, winSL + nb * kWin * 4, base, N);
@@ -402,7 +407,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     prefetch();
     vmWaitImm<0>();
 
-    unsigned long long cyc[8] = {0, 0, 0, 0, 0, 0, 0, 0}; /* load wait, neighbours, evaluate, commit, row, blocked */
+    /* load wait, neighbours, evaluate, commit, row, blocked; within neighbours:
+     * replay and registration, row wait, prefetch issue; within the row:
+     * the pending row's application, the next row's DMA issue */
+    unsigned long long cyc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     (void)cyc;
     while (k < N) {
         SQ_CYC(c0);
@@ -410,7 +418,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         vmWait(afterGroup); /* the last prefetch group has landed */
         SQ_CYC(c1);
         SQ_CYC_ADD(cyc[0], c0, c1);
-        int limit = N - k;
+        int limit = uni(N - k);
         uint32_t s = ringX[(k + lane) & (kRing - 1)];
         const real u = ringU[(k + lane) & (kRing - 1)];
         if (k + lane >= N)
@@ -436,16 +444,16 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 const long long nN = (long long)nbr[nb] * N;
                 const int Kk = k + delta[nb];
                 for (;;) {
-                    const int base = nbase[nb];
+                    const int base = uni(nbase[nb]);
                     const int e0 = base + lane, e1 = base + kWave + lane;
                     const uint32_t d0 = nd[nb][0], d1 = nd[nb][1];
                     const bool dec0 = (d0 >> 1) == tag, dec1 = (d1 >> 1) == tag;
                     const unsigned long long u0 = __ballot(e0 < N && !dec0);
                     const unsigned long long u1 = __ballot(e1 < N && !dec1);
                     const int winLen = N - base < 2 * kWave ? N - base : 2 * kWave;
-                    const int pfx = u0 ? firstLane(u0) : (u1 ? kWave + firstLane(u1) : winLen);
-                    const int from = applied[nb];
-                    const int applyEnd = base + pfx < Kk ? base + pfx : Kk;
+                    const int pfx = uni(u0 ? firstLane(u0) : (u1 ? kWave + firstLane(u1) : winLen));
+                    const int from = uni(applied[nb]);
+                    const int applyEnd = uni(base + pfx < Kk ? base + pfx : Kk);
                     const bool x0 = e0 >= from && e0 < applyEnd && (d0 & 1u);
                     const bool x1 = e1 >= from && e1 < applyEnd && (d1 & 1u);
                     if (__ballot(x0 || x1)) {
@@ -454,8 +462,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                         if (x1)
                             atomicXor(&Mir[nb][ns[nb][1] >> 5], 1u << (ns[nb][1] & 31));
                     }
-                    if (applyEnd > applied[nb])
-                        applied[nb] = applyEnd;
+                    applied[nb] = uni(applyEnd > from ? applyEnd : from);
                     if (pfx == 2 * kWave && applyEnd == base + 2 * kWave && applyEnd < Kk) {
                         /* whole window replayed and still behind: fetch the next one */
                         ldsWait();
@@ -467,13 +474,17 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                     }
                     break;
                 }
-                const int A = applied[nb];
-                const int base = nbase[nb];
+                const int A = uni(applied[nb]);
+                const int base = uni(nbase[nb]);
                 const int e0 = base + lane, e1 = base + kWave + lane;
-                /* an entry matters to lane i only if e - Kk < i <= 63; the slot
-                 * keeps the serially first entry, encoded uniquely per (e, nb) */
-                pendSet[nb][0] = e0 >= A && e0 < N && e0 - Kk < kWave - 1;
-                pendSet[nb][1] = e1 >= A && e1 < N && e1 - Kk < kWave - 1;
+                /* an entry matters to lane i only if e - Kk < i <= 63, and only
+                 * if it may flip: a decided rejection leaves the spin as the
+                 * mirror has it.  The slot keeps the serially first entry,
+                 * encoded uniquely per (e, nb). */
+                const bool keep0 = (nd[nb][0] >> 1) != tag || (nd[nb][0] & 1u);
+                const bool keep1 = (nd[nb][1] >> 1) != tag || (nd[nb][1] & 1u);
+                pendSet[nb][0] = keep0 && e0 >= A && e0 < N && e0 - Kk < kWave - 1;
+                pendSet[nb][1] = keep1 && e1 >= A && e1 < N && e1 - Kk < kWave - 1;
                 if (pendSet[nb][0])
                     pendShared[nb][0] = atomicMin(&tbl[ns[nb][0] & tblMask], pendCode(e0 - Kk, N, nb)) != kPendEmpty;
                 if (pendSet[nb][1])
@@ -482,10 +493,11 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 const int winEnd = base + 2 * kWave < N ? base + 2 * kWave : N;
                 if (winEnd < N) {
                     int lim = winEnd - Kk + 1;
-                    limit = lim < limit ? lim : limit;
+                    limit = uni(lim < limit ? lim : limit);
                 }
             }
         }
+        SQ_CYC(n1);
         waveSync();
         /* a slot written by more than one entry is marked (every registration
          * after the first found it occupied): lanes meeting it wait instead
@@ -503,6 +515,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 waveSync();
             }
         }
+        SQ_CYC(n2);
         /* the staged row has had the whole neighbour phase to arrive: wait for
          * it before the next windows are requested (vmcnt counts in order) */
         real rowCorr = 0;
@@ -523,9 +536,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
          * ended: request them now, so the DMA overlaps evaluation and commit */
         bool pendSetNow[2][2] = {{pendSet[0][0], pendSet[0][1]}, {pendSet[1][0], pendSet[1][1]}};
         uint32_t pendSite[2][2] = {{ns[0][0], ns[0][1]}, {ns[1][0], ns[1][1]}};
+        SQ_CYC(n3);
         prefetch();
         SQ_CYC(c2);
         SQ_CYC_ADD(cyc[1], c1, c2);
+        SQ_CYC_ADD(cyc[6], c1, n1);
+        SQ_CYC_ADD(cyc[7], n2, n3);
+        SQ_CYC_ADD(cyc[8], n3, c2);
 
         /* evaluate attempt k + lane speculatively */
         const bool valid = lane < limit;
@@ -582,7 +599,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 blocked = true;
         }
         const unsigned long long stopMask = __ballot(!valid || blocked);
-        const int firstStop = firstLane(stopMask);
+        const int firstStop = uni(firstLane(stopMask));
         int c = firstStop, flipLane = -1;
         if constexpr (PLAIN) {
             /* reference algorithm: field = J[s] . q_y recomputed per attempt,
@@ -629,7 +646,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         } else {
             const bool acc = accLocal;
             const unsigned long long accMask = __ballot(valid && !blocked && acc);
-            const int firstAcc = firstLane(accMask);
+            const int firstAcc = uni(firstLane(accMask));
             if (firstAcc < firstStop) {
                 c = firstAcc + 1;
                 flipLane = firstAcc;
@@ -714,6 +731,8 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         if constexpr (kDefer)
             if (rowPending)
                 applyPendingRow();
+        SQ_CYC(r1);
+        SQ_CYC_ADD(cyc[9], c4, r1);
         if (flipLane >= 0) {
             const uint32_t x = __shfl(s, flipLane);
             const bool qx = __shfl((int)qpos, flipLane) != 0;
@@ -773,7 +792,9 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 }
             }
         }
-        k += c;
+        SQ_CYC(r2);
+        SQ_CYC_ADD(cyc[10], r1, r2);
+        k = uni(k + c);
         waveSync();
         SQ_CYC(c5);
         SQ_CYC_ADD(cyc[3], c3, c5);
@@ -802,7 +823,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         atomicAdd(&a.counters[1], nBatches);
         atomicAdd(&a.counters[2], nBlocked);
 #ifdef SQAOD_SWEEP_CYCLES
-        for (int t = 0; t < 6; ++t)
+        for (int t = 0; t < 11; ++t)
             atomicAdd(&a.counters[3 + t], cyc[t]);
 #endif
     }
@@ -812,7 +833,7 @@ template <class real, int MODE>
 __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
                                                                                 int xcdRemap) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int w = threadIdx.x / kWave;
+    const int w = uni(threadIdx.x / kWave); /* the wave index, uniform per wave */
 
     /* XCD-aware placement: consecutive slices on the same XCD (blocks are
      * dispatched round-robin over the 8 XCDs). */
@@ -854,7 +875,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepBatchKern
                                                                                      int *queue, int regionBytes) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
+    const int w = uni(threadIdx.x / kWave); /* the wave index, uniform per wave */
     uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
     unsigned char *region = lds + kExpTabBytes + (size_t)regionBytes * w;
     if (threadIdx.x < 32)
@@ -873,7 +894,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepBatchKern
         for (int i = lane; i < a.Np; i += kWave)
             hS[i] = a.h[i];
         waveSync();
-        sweepSlice<real, MODE>(a, (int)it.y, expTab, hS, region + (size_t)a.Np * sizeof(real));
+        sweepSlice<real, MODE>(a, uni((int)it.y), expTab, hS, region + (size_t)a.Np * sizeof(real));
         waveSync();
     }
 }
