@@ -329,12 +329,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
      * of 256; the tail round covers what is left) */
     auto applyPendingRow = [&]() {
         constexpr int kR = 4;
-        for (int j0 = lane * 4; j0 < Np; j0 += kR * kWave * 4) {
+        /* b0 and the slice bounds are wave-uniform: scalar loop and branches */
+        for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
             real fv[kR][4], sv[kR][4];
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
-                const int j = j0 + r * kWave * 4;
-                if (j < Np) {
+                const int j = b0 + r * kWave * 4 + lane * 4;
+                if (b0 + r * kWave * 4 < Np) {
                     ldsLoad4(F + j, fv[r]);
                     if constexpr (FP8)
                         fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(stage8 + j), sv[r]);
@@ -344,8 +345,8 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             }
 #pragma unroll
             for (int r = 0; r < kR; ++r) {
-                const int j = j0 + r * kWave * 4;
-                if (j < Np) {
+                const int j = b0 + r * kWave * 4 + lane * 4;
+                if (b0 + r * kWave * 4 < Np) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
                         fv[r][t] = fv[r][t] + rowD * sv[r][t];
@@ -616,7 +617,8 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 }
                 const int nDots = firstStop - g < kDotGroup ? firstStop - g : kDotGroup;
 #pragma unroll 4
-                for (int j = lane * 4; j < Np; j += kWave * 4) {
+                for (int b = 0; b < Np; b += kWave * 4) { /* uniform bound: a scalar loop */
+                    const int j = b + lane * 4;
                     const uint32_t qw = Qb[j >> 5] >> (j & 31);
 #pragma unroll
                     for (int t = 0; t < kDotGroup; ++t) {
@@ -639,7 +641,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 const bool inGroup = lane >= g && lane < g + nDots;
                 const unsigned long long accMask = __ballot(inGroup && decide(f));
                 if (accMask)
-                    flipLane = firstLane(accMask);
+                    flipLane = uni(firstLane(accMask));
             }
             if (flipLane >= 0)
                 c = flipLane + 1;
