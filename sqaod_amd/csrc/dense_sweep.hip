@@ -298,17 +298,24 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     const int posY = slicePos(y, m);
     const int delta[2] = {slicePos(n0, m) < posY ? 1 : 0, slicePos(n1, m) < posY ? 1 : 0};
 
-    if (!PLAIN)
-        for (int i = lane; i < Np; i += kWave)
-            F[i] = a.F[(long long)y * Np + i];
-    for (int i = lane; i < tblN; i += kWave)
-        tbl[i] = kPendEmpty;
-    for (int i = lane; i < Nw; i += kWave) {
-        Qb[i] = a.Qbits[(long long)y * Nw + i];
-        Mir[0][i] = a.Qbits[(long long)n0 * Nw + i];
-        Mir[1][i] = a.Qbits[(long long)n1 * Nw + i];
+    /* start-up state by DMA, all in flight together (the first batch waits
+     * for it): fields (1 KiB per op), spin bits and the two mirrors */
+    if (!PLAIN) {
+        const unsigned char *src = reinterpret_cast<const unsigned char *>(a.F + (long long)y * Np);
+        const uint32_t FL = winDL + (uint32_t)sweepFixedBytes<real>();
+        for (int j0 = 0; j0 < Np * (int)sizeof(real); j0 += kWave * 16)
+            dmaDwordx4(src + j0 + lane * 16, FL + j0);
     }
-    waveSync();
+    {
+        const uint32_t QbL = ldsAddress(Qb);
+        for (int b = 0; b < Nw; b += kWave) {
+            dmaEntries<4, false, kWave>(a.Qbits + (long long)y * Nw + b, QbL + b * 4, b, Nw);
+            dmaEntries<4, false, kWave>(a.Qbits + (long long)n0 * Nw + b, QbL + (Nw + b) * 4, b, Nw);
+            dmaEntries<4, false, kWave>(a.Qbits + (long long)n1 * Nw + b, QbL + (2 * Nw + b) * 4, b, Nw);
+        }
+    }
+    for (int b = 0; b < tblN; b += kWave)
+        tbl[b + lane] = kPendEmpty;
 
     const uint32_t tag = a.tag;
     int applied[2] = {0, 0};
@@ -814,11 +821,17 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         }
     }
     if (!aborted) {
-        if (!PLAIN)
-            for (int i = lane; i < Np; i += kWave)
-                a.F[(long long)y * Np + i] = F[i];
-        for (int i = lane; i < Nw; i += kWave)
-            a.Qbits[(long long)y * Nw + i] = Qb[i];
+        if (!PLAIN) {
+#pragma unroll 4
+            for (int b = 0; b < Np; b += kWave * 4) {
+                real v[4];
+                ldsLoad4(F + b + lane * 4, v);
+                ldsStore4(a.F + (long long)y * Np + b + lane * 4, v); /* a plain vector store, here to global */
+            }
+        }
+        for (int b = 0; b < Nw; b += kWave)
+            if (b + lane < Nw)
+                a.Qbits[(long long)y * Nw + b + lane] = Qb[b + lane];
     }
     if (lane == 0 && a.counters) {
         atomicAdd(&a.counters[0], nFlips);
@@ -852,8 +865,16 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     unsigned char *p = lds + kExpTabBytes + (size_t)a.Np * sizeof(real) + sweepWaveBytes<real, MODE>(a.Np, a.ld8) * w;
     if (threadIdx.x < 32)
         expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
-    for (int i = threadIdx.x; i < a.Np; i += blockDim.x)
-        hS[i] = a.h[i];
+    {
+        /* h by DMA, the block's waves taking 1 KiB pieces in turn (h holds Np
+         * reals, a multiple of 1 KiB) */
+        const int lane = threadIdx.x & (kWave - 1), nW = uni(blockDim.x / kWave);
+        const unsigned char *src = reinterpret_cast<const unsigned char *>(a.h);
+        const uint32_t hL = ldsAddress(hS);
+        for (int j0 = w * kWave * 16; j0 < a.Np * (int)sizeof(real); j0 += nW * kWave * 16)
+            dmaDwordx4(src + j0 + lane * 16, hL + j0);
+        vmWaitImm<0>();
+    }
     __syncthreads();
     if (y >= a.m)
         return;
@@ -893,9 +914,13 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepBatchKern
         const uint2 it = items[item];
         const SweepArgs<real> a = args[__builtin_amdgcn_readfirstlane((int)it.x)];
         real *hS = reinterpret_cast<real *>(region);
-        for (int i = lane; i < a.Np; i += kWave)
-            hS[i] = a.h[i];
-        waveSync();
+        {
+            const unsigned char *src = reinterpret_cast<const unsigned char *>(a.h);
+            const uint32_t hL = ldsAddress(hS);
+            for (int j0 = 0; j0 < a.Np * (int)sizeof(real); j0 += kWave * 16)
+                dmaDwordx4(src + j0 + lane * 16, hL + j0);
+            vmWaitImm<0>();
+        }
         sweepSlice<real, MODE>(a, uni((int)it.y), expTab, hS, region + (size_t)a.Np * sizeof(real));
         waveSync();
     }
