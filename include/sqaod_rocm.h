@@ -74,8 +74,9 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
 /* annealer_set_preferences (:279-299), one preference per call, value as text:
  * "algorithm" ("coloring" | "sa_naive" | "naive" | "default" | ...),
  * "n_trotters" (int), and the rocm-device knobs "rng" ("philox" | "mt19937"),
- * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2"),
- * "slices_per_block" (int, 0 = auto), "refresh_interval" (int, steps between
+ * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2" | "fp8"),
+ * "slices_per_block" (int, 0 = auto), "sweep_launch" ("auto" | "cooperative"),
+ * "refresh_interval" (int, steps between
  * field recomputes; 0 = only after the spins were set), "refresh_engine"
  * ("mfma" | "vector"), "profile" ("0" | "1"). */
 int sqaod_rocm_dg_annealer_set_preference(sqaod_rocm_dg_annealer *a, const char *name, const char *value);

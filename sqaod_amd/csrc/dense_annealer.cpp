@@ -12,6 +12,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <map>
+#include <mutex>
 #include <strings.h>
 
 #include "solver.h"
@@ -115,7 +117,25 @@ Device::Device(int devno) : devno_(devno) {
         maxLds_ = (size_t)optin;
 }
 
+namespace {
+std::mutex g_sweepersMu;
+std::map<int, int> g_sweepers; /* GPU -> Devices that launch sweeps */
+}  // namespace
+
+bool Device::soleSweeper() {
+    std::lock_guard<std::mutex> lock(g_sweepersMu);
+    if (!sweeper_) {
+        sweeper_ = true;
+        ++g_sweepers[devno_];
+    }
+    return g_sweepers[devno_] == 1;
+}
+
 Device::~Device() {
+    if (sweeper_) {
+        std::lock_guard<std::mutex> lock(g_sweepersMu);
+        --g_sweepers[devno_];
+    }
     (void)hipSetDevice(devno_);
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
@@ -319,6 +339,13 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
         fieldsDirty_ = true;
     } else if (name == "slices_per_block") {
         slicesPerBlock_ = atoi(value.c_str());
+    } else if (name == "sweep_launch") {
+        if (value == "cooperative")
+            coopLaunch_ = true;
+        else if (value == "auto")
+            coopLaunch_ = false;
+        else
+            SQ_THROW("unknown sweep_launch '%s' (auto | cooperative)", value.c_str());
     } else if (name == "refresh_interval") {
         refreshInterval_ = atoi(value.c_str());
     } else if (name == "profile") {
@@ -352,6 +379,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
                                       : fieldFormat_ == fieldFP8  ? "fp8"
                                                                   : (sizeof(real) == 4 ? "fp32" : "fp64"));
     p.emplace_back("slices_per_block", std::to_string(slicesPerBlock_));
+    p.emplace_back("sweep_launch", coopLaunch_ ? "cooperative" : "auto");
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
     p.emplace_back("refresh_engine", refreshVector_ ? "vector" : "mfma");
     p.emplace_back("profile", std::to_string(profile_));
@@ -727,7 +755,8 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
                 denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_));
     if (profile_)
         phaseBegin(2, pe);
-    denseSweep<real>(a, plainSweep_, fieldFormat_, S, dev_->stream());
+    denseSweep<real>(a, plainSweep_, fieldFormat_, S, !coopLaunch_ && dev_->soleSweeper(), dev_->numCUs(),
+                     dev_->stream());
     if (profile_)
         phaseEnd(pe);
     if (pending_.size() > 3000)

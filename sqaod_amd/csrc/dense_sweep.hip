@@ -26,11 +26,15 @@
  * attempt and touch the same site block that lane (and the batch is cut
  * there).  Every wait is on a serially earlier attempt, so the waits form no
  * cycle and all waves finish; a spin limit turns a bug into an error instead
- * of a hang.  All waves must be co-resident: the launch is cooperative.
+ * of a hang.  All waves must be co-resident (see denseSweep's launch).
  */
 #include "common.h"
 #include "kernels.h"
 #include "sq_expf.h"
+
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace sqaod_rocm {
 
@@ -1021,8 +1025,33 @@ template void denseSweepBatch<float>(const SweepArgs<float> *, const uint2 *, in
 template void denseSweepBatch<double>(const SweepArgs<double> *, const uint2 *, int, int *, bool, int, size_t, int,
                                       int, hipStream_t);
 
+/* Whether every block of a launch of fn fits on the device at once. */
+static bool fitsAtOnce(const void *fn, int threads, size_t lds, int nBlocks, int numCUs) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void *, int, size_t>, int> perCU;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(fn, threads, lds);
+    auto it = perCU.find(key);
+    if (it == perCU.end()) {
+        int n = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, lds));
+        it = perCU.emplace(key, n).first;
+    }
+    return (long long)it->second * numCUs >= nBlocks;
+}
+
+/* The sweep's waves spin on each other, so all of them must be resident.
+ * A cooperative launch guarantees that, at the price of the runtime's queue
+ * synchronisation around it (about 20 us per launch here, 5% of a step at
+ * N=4096, m=256).  A direct launch is used instead when allowDirect (the
+ * caller's stream is the only one on this GPU launching sweeps in this
+ * process) and every block fits on the device at once: other kernels that
+ * hold CUs finish, so the dispatched blocks all become resident.  If they
+ * cannot (another process's spinning kernels), the spin limit turns the wait
+ * into an error. */
 template <class real>
-void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, hipStream_t stream) {
+void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, bool allowDirect,
+                int numCUs, hipStream_t stream) {
     SQ_THROW_IF(args.Np % 256 != 0, "padded N must be a multiple of 256");
     SQ_THROW_IF(slicesPerBlock < 1 || slicesPerBlock > kMaxSlicesPerBlock, "slices per block out of range (1..%d)",
                 kMaxSlicesPerBlock);
@@ -1040,11 +1069,14 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SweepArgs<real> a = args;
     void *kargs[] = {(void *)&a, (void *)&slicesPerBlock, (void *)&xcdRemap};
-    HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(nBlocks), dim3(slicesPerBlock * kWave), kargs, (unsigned)lds,
-                                         stream));
+    if (allowDirect && fitsAtOnce(fn, slicesPerBlock * kWave, lds, nBlocks, numCUs))
+        HIP_CHECK(hipLaunchKernel(fn, dim3(nBlocks), dim3(slicesPerBlock * kWave), kargs, (unsigned)lds, stream));
+    else
+        HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(nBlocks), dim3(slicesPerBlock * kWave), kargs, (unsigned)lds,
+                                             stream));
 }
 
-template void denseSweep<float>(const SweepArgs<float> &, bool, int, int, hipStream_t);
-template void denseSweep<double>(const SweepArgs<double> &, bool, int, int, hipStream_t);
+template void denseSweep<float>(const SweepArgs<float> &, bool, int, int, bool, int, hipStream_t);
+template void denseSweep<double>(const SweepArgs<double> &, bool, int, int, bool, int, hipStream_t);
 
 }  // namespace sqaod_rocm

@@ -231,7 +231,8 @@ struct SweepArgs {
  * and updates them with the rows of args.J (fp32 / bf16x2 forms) or of
  * args.J8 (fieldFormat == fieldFP8, float solvers). */
 template <class real>
-void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, hipStream_t stream);
+void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int slicesPerBlock, bool allowDirect,
+                int numCUs, hipStream_t stream);
 
 /* Bytes of LDS one block of the local-field sweep needs (ld8: the fp8 row
  * stride, used when fieldFormat == fieldFP8). */

@@ -53,10 +53,15 @@ public:
     size_t maxLdsPerBlock() const { return maxLds_; }
     void makeCurrent() const;
     void synchronize() const;
+    /* Registers this Device (its stream) as one that launches spin-waiting
+     * sweeps and returns whether it is the only such Device of its GPU in this
+     * process: then no sweep of another stream can run beside its sweeps. */
+    bool soleSweeper();
 
 private:
     int devno_;
     hipStream_t stream_;
+    bool sweeper_ = false;
     int numCUs_ = 0;
     size_t maxLds_ = 65536;
 };
@@ -205,6 +210,7 @@ private:
     bool plainSweep_ = false;     /* "sweep": "local" (default) | "plain" */
     int fieldFormat_ = fieldF32;  /* "field_precision": "fp32" | "bf16x2" | "fp8" */
     int slicesPerBlock_ = 0;      /* "slices_per_block": 0 = automatic */
+    bool coopLaunch_ = false;     /* "sweep_launch": cooperative always, or direct when safe (auto) */
     int refreshInterval_ = 1;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
     bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
     int spinLimit_ = 1 << 20;

@@ -44,7 +44,8 @@ def q_of(an):
 
 
 @pytest.mark.parametrize("variant", [dict(), dict(field_precision="bf16x2"), dict(field_precision="fp8"),
-                                     dict(sweep="plain"), dict(slices_per_block=1)])
+                                     dict(sweep="plain"), dict(slices_per_block=1),
+                                     dict(sweep_launch="cooperative")])
 def test_config1_bit_exact_trajectory(oracle, rocm, variant):
     """N=128, m=32, integer W in [-8, 8], G 5.0 -> 0.01 (x0.97), beta=50: the
     spins of every slice and the energies equal the serial CPU annealer's
