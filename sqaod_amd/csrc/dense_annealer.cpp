@@ -744,10 +744,14 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     SweepArgs<real> a = makeSweepArgs(G, beta);
     int S = slicesPerBlock_;
     if (S <= 0) {
-        S = 4;
-        while (S > 1 && (denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) >
-                             dev_->maxLdsPerBlock() ||
-                         S > m_))
+        /* one slice per workgroup while the CUs last (a CU to each wave is
+         * faster than sharing one), more when m exceeds the CU count, as LDS
+         * allows */
+        S = 1;
+        while (S < 4 && divru(m_, S) > dev_->numCUs())
+            S <<= 1;
+        while (S > 1 && denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) >
+                            dev_->maxLdsPerBlock())
             S >>= 1;
     }
     SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) > dev_->maxLdsPerBlock(),

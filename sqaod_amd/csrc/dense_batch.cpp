@@ -153,6 +153,7 @@ void GroupedDenseStepper::run(Device &dev, const std::vector<DenseGraphAnnealer<
     SQ_THROW_IF(inst.empty() || order.empty(), "grouped step: no instances");
     SQ_THROW_IF(nSteps <= 0, "no steps");
     dev.makeCurrent();
+    (void)dev.soleSweeper(); /* its persistent sweeps spin: registered, so other Devices' sweeps launch cooperatively */
     const int nI = (int)inst.size();
     int maxNp = 0, maxLd8 = 0, maxM = 0;
     std::vector<uint2> items;
