@@ -131,34 +131,43 @@ __device__ __forceinline__ void refreshBF16x2Block(const uint4 *frag, const uint
             }
         };
         /* A operands of four tiles are gathered (lane shuffle, then table
-         * read) before their MFMAs, so the LDS round trips of a group overlap;
-         * a pass with all 16 tiles in range has no per-tile branch at all */
+         * read) one group ahead of the MFMAs that use them, so a group's LDS
+         * round trips run under the previous group's MFMAs; a pass with all
+         * 16 tiles in range has no per-tile branch at all */
         const bool fullPass = y0 + kRowsPerPass <= m;
-        auto compute = [&](int buf) {
+        constexpr int kGroups = kTiles / 4;
+        auto groupLive = [&](int t0) { return fullPass || y0 + 16 * t0 < m; }; /* wave-uniform */
+        auto gather = [&](int buf, int step, uint4 a4[4]) {
+            const int d = step / kGroups, t0 = (step % kGroups) * 4;
+            if (!groupLive(t0))
+                return;
+            const uint4 q4 = qw[buf][t0 >> 2];
+            const uint32_t qd = d == 0 ? q4.x : d == 1 ? q4.y : d == 2 ? q4.z : q4.w;
 #pragma unroll
-            for (int d = 0; d < kDepth; ++d) {
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t word = (uint32_t)__shfl((int)qd, 16 * u + c, kWave);
+                a4[u] = lut[(word >> (8 * g)) & 0xffu];
+            }
+        };
+        auto compute = [&](int buf) {
+            uint4 a4[2][4];
+            gather(buf, 0, a4[0]);
+#pragma unroll
+            for (int step = 0; step < kDepth * kGroups; ++step) {
+                if (step + 1 < kDepth * kGroups)
+                    gather(buf, step + 1, a4[(step + 1) & 1]);
+                const int d = step / kGroups, t0 = (step % kGroups) * 4;
+                if (!groupLive(t0))
+                    continue;
                 const bf16x8 bhi = __builtin_bit_cast(bf16x8, bh[buf][d]);
                 const bf16x8 blo = __builtin_bit_cast(bf16x8, bl[buf][d]);
 #pragma unroll
-                for (int t0 = 0; t0 < kTiles; t0 += 4) {
-                    if (!fullPass && y0 + 16 * t0 >= m) /* wave-uniform */
-                        break;
-                    const uint4 q4 = qw[buf][t0 >> 2];
-                    const uint32_t qd = d == 0 ? q4.x : d == 1 ? q4.y : d == 2 ? q4.z : q4.w;
-                    uint4 a4[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * u + c, kWave);
-                        a4[u] = lut[(word >> (8 * g)) & 0xffu];
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int t = t0 + u;
-                        if (fullPass || y0 + 16 * t < m) {
-                            const bf16x8 a = __builtin_bit_cast(bf16x8, a4[u]);
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bhi, acc[t], 0, 0, 0);
-                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, blo, acc[t], 0, 0, 0);
-                        }
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u;
+                    if (fullPass || y0 + 16 * t < m) {
+                        const bf16x8 av = __builtin_bit_cast(bf16x8, a4[step & 1][u]);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bhi, acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, blo, acc[t], 0, 0, 0);
                     }
                 }
             }
