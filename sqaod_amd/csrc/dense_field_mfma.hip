@@ -354,23 +354,40 @@ __device__ __forceinline__ void refreshFP8Block(const uint4 *frag, const float *
                 qv[buf][t] = (y < m) ? v : uint2{0u, 0u};
             }
         };
-        auto compute = [&](int buf) {
+        /* A operands (four table reads per tile) are gathered one group of
+         * four tiles ahead of the MFMAs that use them */
+        constexpr int kGroups = kTiles / 4;
+        auto gather = [&](int buf, int step, i32x8 aop[4]) {
+            const int d = step / kGroups, t0 = (step % kGroups) * 4;
 #pragma unroll
-            for (int d = 0; d < kDepth8; ++d) {
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u;
+                if (y0 + 16 * t < m) { /* wave-uniform */
+                    const uint32_t word = d == 0 ? qv[buf][t].x : qv[buf][t].y;
+                    const uint2 l0 = lut[word & 0xffu], l1 = lut[(word >> 8) & 0xffu];
+                    const uint2 l2 = lut[(word >> 16) & 0xffu], l3 = lut[word >> 24];
+                    aop[u] = i32x8{(int)l0.x, (int)l0.y, (int)l1.x, (int)l1.y,
+                                   (int)l2.x, (int)l2.y, (int)l3.x, (int)l3.y};
+                }
+            }
+        };
+        auto compute = [&](int buf) {
+            i32x8 aop[2][4];
+            gather(buf, 0, aop[0]);
+#pragma unroll
+            for (int step = 0; step < kDepth8 * kGroups; ++step) {
+                if (step + 1 < kDepth8 * kGroups)
+                    gather(buf, step + 1, aop[(step + 1) & 1]);
+                const int d = step / kGroups, t0 = (step % kGroups) * 4;
                 const uint4 x0 = b0[buf][d], x1 = b1[buf][d];
                 const i32x8 bop = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w,
                                    (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
 #pragma unroll
-                for (int t = 0; t < kTiles; ++t) {
-                    if (y0 + 16 * t < m) { /* wave-uniform */
-                        const uint32_t word = d == 0 ? qv[buf][t].x : qv[buf][t].y;
-                        const uint2 l0 = lut[word & 0xffu], l1 = lut[(word >> 8) & 0xffu];
-                        const uint2 l2 = lut[(word >> 16) & 0xffu], l3 = lut[word >> 24];
-                        const i32x8 aop = {(int)l0.x, (int)l0.y, (int)l1.x, (int)l1.y,
-                                           (int)l2.x, (int)l2.y, (int)l3.x, (int)l3.y};
-                        acc[t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aop, bop, acc[t], 0, 0, 0,
-                                                                                  kE4M3Unit, 0, kE4M3Unit);
-                    }
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u;
+                    if (y0 + 16 * t < m) /* wave-uniform */
+                        acc[t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(aop[step & 1][u], bop, acc[t], 0, 0,
+                                                                                  0, kE4M3Unit, 0, kE4M3Unit);
                 }
             }
         };
