@@ -361,7 +361,7 @@ void BipartiteGraphAnnealer<real>::computeFields(int side) {
             const DevBuf<uint16_t> &f = side ? d_fragJ16_ : d_fragJT16_;
             fieldRefreshBF16x2(f.ptr, f.n, Qin, nOutP, K, m_, NwIn, ldE, (float *)d_dE_.ptr, dev_->stream());
         } else {
-            const DevBuf<float> &f = side ? d_fragJ32_ : d_fragJT32_;
+            const DevBuf<uint16_t> &f = side ? d_fragJ32_ : d_fragJT32_;
             fieldRefreshF32MFMA(f.ptr, f.n, Qin, nOutP, K, m_, NwIn, ldE, (float *)d_dE_.ptr, dev_->stream());
         }
     } else {

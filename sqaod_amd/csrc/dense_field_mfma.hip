@@ -8,7 +8,9 @@
  *
  *  - bf16x2: J = hi + lo (two bf16 planes), spins +-1 exact in bf16,
  *    v_mfma_f32_16x16x32_bf16 twice per k-block, fp32 accumulation.
- *  - fp32:   v_mfma_f32_16x16x4_f32, spins as +-1.0f.
+ *  - fp32:   J = hi + mid + lo exactly (three bf16 planes, bf16Split3), the
+ *    same kernel with three MFMAs per k-block: every product is exact and
+ *    accumulates in fp32, at a third of the v_mfma_f32_16x16x4_f32 cost.
  *  - fp8:    J as e4m3 bytes with a power-of-two scale per column, spins
  *    +-1 exact in e4m3, v_mfma_scale_f32_16x16x128_f8f6f4 with unit block
  *    scales; the column scale multiplies the reduced sum (exactly).
@@ -90,24 +92,26 @@ __device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int
 
 typedef float RedTile[kTiles * 4][kWave];
 
-/* frag: [column blocks][K/32 k-blocks][2 planes][64 lanes] x 16 B; the block
- * writes columns [16 cb, 16 cb + 16) of F (row stride ldF).  lut: byte -> 8
- * spins as bf16 +-1, filled by the caller. */
-__device__ __forceinline__ void refreshBF16x2Block(const uint4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
-                                                   int ldF, float *F, int cb, RedTile *red, const uint4 *lut) {
+/* frag: [column blocks][K/32 k-blocks][PLANES][64 lanes] x 16 B (PLANES bf16
+ * planes whose sum is B: hi + lo for the bf16x2 form, hi + mid + lo -- exactly
+ * B -- for the fp32 form); the block writes columns [16 cb, 16 cb + 16) of F
+ * (row stride ldF).  lut: byte -> 8 spins as bf16 +-1, filled by the caller. */
+template <int PLANES>
+__device__ __forceinline__ void refreshBF16Block(const uint4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
+                                                 int ldF, float *F, int cb, RedTile *red, const uint4 *lut) {
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
     const int nK = K / 32;      /* k-blocks of 32; K is a multiple of 256 */
     const int nS = nK / kDepth; /* super-blocks */
     const int sOff = (int)(((long long)cb * 3) % nS);
-    const uint4 *fb = frag + (size_t)cb * nK * 2 * kWave + lane;
+    const uint4 *fb = frag + (size_t)cb * nK * PLANES * kWave + lane;
     for (int y0 = 0; y0 < m; y0 += kRowsPerPass) {
         f32x4 acc[kTiles];
 #pragma unroll
         for (int t = 0; t < kTiles; ++t)
             acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        uint4 bh[2][kDepth], bl[2][kDepth], qw[2][kTiles / 4];
+        uint4 bp[PLANES][2][kDepth], qw[2][kTiles / 4];
         /* loads super-block l (logical index) into buffer buf; l < nS */
         auto fetch = [&](int buf, int l) {
             int s = l + sOff;
@@ -116,9 +120,10 @@ __device__ __forceinline__ void refreshBF16x2Block(const uint4 *frag, const uint
             const int kb0 = s * kDepth;
 #pragma unroll
             for (int d = 0; d < kDepth; ++d) {
-                const uint4 *p = fb + (size_t)(kb0 + d) * 2 * kWave;
-                bh[buf][d] = p[0];
-                bl[buf][d] = p[kWave];
+                const uint4 *p = fb + (size_t)(kb0 + d) * PLANES * kWave;
+#pragma unroll
+                for (int pl = 0; pl < PLANES; ++pl)
+                    bp[pl][buf][d] = p[pl * kWave];
             }
             /* words kb0 .. kb0+3 of row y0 + 64 r + lane (kb0 is a multiple of 4, Nw of 8);
              * rows past m read row m - 1 and are zeroed, so every load is
@@ -159,15 +164,16 @@ __device__ __forceinline__ void refreshBF16x2Block(const uint4 *frag, const uint
                 const int d = step / kGroups, t0 = (step % kGroups) * 4;
                 if (!groupLive(t0))
                     continue;
-                const bf16x8 bhi = __builtin_bit_cast(bf16x8, bh[buf][d]);
-                const bf16x8 blo = __builtin_bit_cast(bf16x8, bl[buf][d]);
+
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int t = t0 + u;
                     if (fullPass || y0 + 16 * t < m) {
                         const bf16x8 av = __builtin_bit_cast(bf16x8, a4[step & 1][u]);
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bhi, acc[t], 0, 0, 0);
-                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, blo, acc[t], 0, 0, 0);
+#pragma unroll
+                        for (int pl = 0; pl < PLANES; ++pl)
+                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                av, __builtin_bit_cast(bf16x8, bp[pl][buf][d]), acc[t], 0, 0, 0);
                     }
                 }
             }
@@ -205,100 +211,17 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x2Kernel(const 
     __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
     __shared__ uint4 lut[256];                        /* 4 KiB */
     fillLutBF16(lut);
-    refreshBF16x2Block(frag, Qbits, K, m, Nw, ldF, F, blockIdx.x, red, lut);
+    refreshBF16Block<2>(frag, Qbits, K, m, Nw, ldF, F, blockIdx.x, red, lut);
 }
 
-/* frag: [column blocks][K/16 k-blocks][64 lanes] x float4 */
-__device__ __forceinline__ void refreshF32Block(const float4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
-                                                int ldF, float *F, int cb, RedTile *red) {
-    const int lane = threadIdx.x % kWave;
-    const int w = threadIdx.x / kWave;
-    const int c = lane & 15, g = lane >> 4;
-    const int nK = K / 16; /* k-blocks of 16 (four 16x16x4 steps) */
-    const int nS = nK / kDepth;
-    const int sOff = (int)(((long long)cb * 3) % nS);
-    const float4 *fb = frag + (size_t)cb * nK * kWave + lane;
-    for (int y0 = 0; y0 < m; y0 += kRowsPerPass) {
-        f32x4 acc[kTiles];
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t)
-            acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        float4 bv[2][kDepth];
-        uint2 qw[2][kTiles / 4];
-        auto fetch = [&](int buf, int l) {
-            int s = l + sOff;
-            if (s >= nS)
-                s -= nS;
-            const int kb0 = s * kDepth;
-#pragma unroll
-            for (int d = 0; d < kDepth; ++d)
-                bv[buf][d] = fb[(size_t)(kb0 + d) * kWave];
-            /* a super-block spans 64 k: words kb0/2, kb0/2 + 1 (kb0 a multiple of 4) */
-#pragma unroll
-            for (int r = 0; r < kTiles / 4; ++r) {
-                const int y = y0 + 64 * r + lane;
-                const uint2 v = *reinterpret_cast<const uint2 *>(Qbits + (size_t)(y < m ? y : m - 1) * Nw + kb0 / 2);
-                qw[buf][r] = (y < m) ? v : uint2{0u, 0u};
-            }
-        };
-        const bool fullPass = y0 + kRowsPerPass <= m;
-        auto compute = [&](int buf) {
-#pragma unroll
-            for (int d = 0; d < kDepth; ++d) {
-                const float b[4] = {bv[buf][d].x, bv[buf][d].y, bv[buf][d].z, bv[buf][d].w};
-#pragma unroll
-                for (int t0 = 0; t0 < kTiles; t0 += 4) {
-                    if (!fullPass && y0 + 16 * t0 >= m) /* wave-uniform */
-                        break;
-                    const uint2 q2 = qw[buf][t0 >> 2];
-                    const uint32_t qd = (d >> 1) ? q2.y : q2.x;
-                    uint32_t nib[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint32_t word = (uint32_t)__shfl((int)qd, 16 * u + c, kWave);
-                        /* lane group g holds k = 16 kb + 4 g + s for step s (same for A and B) */
-                        nib[u] = word >> (16 * (d & 1) + 4 * g);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int t = t0 + u;
-                        if (fullPass || y0 + 16 * t < m) {
-#pragma unroll
-                            for (int s = 0; s < 4; ++s) {
-                                const float a = ((nib[u] >> s) & 1u) ? 1.f : -1.f;
-                                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s], acc[t], 0, 0, 0);
-                            }
-                        }
-                    }
-                }
-            }
-        };
-        for (int l = w; l < nS; l += 2 * kSplitK) {
-            if (l == w)
-                fetch(0, l);
-            if (l + kSplitK < nS)
-                fetch(1, l + kSplitK);
-            compute(0);
-            if (l + kSplitK >= nS)
-                break;
-            if (l + 2 * kSplitK < nS)
-                fetch(0, l + 2 * kSplitK);
-            compute(1);
-        }
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                red[w][4 * t + r][lane] = acc[t][r];
-        reduceStore(red, y0, m, cb, ldF, F);
-    }
+__global__ __launch_bounds__(kBlockThreads) void fieldRefreshBF16x3Kernel(const uint4 *frag, const uint32_t *Qbits,
+                                                                          int K, int m, int Nw, int ldF, float *F) {
+    __shared__ float red[kSplitK][kTiles * 4][kWave]; /* 64 KiB */
+    __shared__ uint4 lut[256];                        /* 4 KiB */
+    fillLutBF16(lut);
+    refreshBF16Block<3>(frag, Qbits, K, m, Nw, ldF, F, blockIdx.x, red, lut);
 }
 
-__global__ __launch_bounds__(kBlockThreads) void fieldRefreshF32Kernel(const float4 *frag, const uint32_t *Qbits,
-                                                                       int K, int m, int Nw, int ldF, float *F) {
-    __shared__ float red[kSplitK][kTiles * 4][kWave];
-    refreshF32Block(frag, Qbits, K, m, Nw, ldF, F, blockIdx.x, red);
-}
 
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 constexpr int kDepth8 = 2; /* fp8: k-blocks of 128 per super-block (256 k = 8 spin words) */
@@ -452,14 +375,16 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshGroupedKernel(const
     const int cb = (int)blockIdx.x - blockStart[j];
     if constexpr (FORM == fieldBF16x2) {
         fillLutBF16(lut);
-        refreshBF16x2Block(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
-                           red, lut);
+        refreshBF16Block<2>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
+                            red, lut);
     } else if constexpr (FORM == fieldFP8) {
         fillLutFP8(reinterpret_cast<uint2 *>(lut));
         refreshFP8Block(reinterpret_cast<const uint4 *>(jb.frag), jb.colScale, jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
                         jb.F, cb, red, reinterpret_cast<const uint2 *>(lut));
     } else {
-        refreshF32Block(reinterpret_cast<const float4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb, red);
+        fillLutBF16(lut);
+        refreshBF16Block<3>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
+                            red, lut);
     }
 }
 
@@ -488,7 +413,8 @@ __global__ void buildFragFP8Kernel(const uint8_t *B8, int rows, int ld8, int nCo
 
 /* One thread per (column block, k-block, lane): the lane's 8 k values of its
  * column (a row of B, row stride ld, zero past `rows`), split into bf16 hi and lo. */
-__global__ void buildFragBF16x2Kernel(const float *B, int rows, int ld, int nColBlocks, int K, uint4 *frag) {
+template <int PLANES>
+__global__ void buildFragBF16Kernel(const float *B, int rows, int ld, int nColBlocks, int K, uint4 *frag) {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nK = K / 32;
     if (idx >= (size_t)nColBlocks * nK * kWave)
@@ -497,34 +423,27 @@ __global__ void buildFragBF16x2Kernel(const float *B, int rows, int ld, int nCol
     const size_t rest = idx / kWave;
     const int kb = (int)(rest % nK), cb = (int)(rest / nK);
     const int col = cb * 16 + (lane & 15), k0 = kb * 32 + 8 * (lane >> 4);
-    uint32_t hw[4] = {0u, 0u, 0u, 0u}, lw[4] = {0u, 0u, 0u, 0u};
+    uint32_t pw[PLANES][4] = {};
     if (col < rows) {
         const float4 *src = reinterpret_cast<const float4 *>(B + (size_t)col * ld + k0);
         const float4 v0 = src[0], v1 = src[1];
         const float v[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            uint16_t hi, lo;
-            bf16Split(v[e], hi, lo);
-            hw[e >> 1] |= (uint32_t)hi << (16 * (e & 1));
-            lw[e >> 1] |= (uint32_t)lo << (16 * (e & 1));
+            uint16_t h[3];
+            if constexpr (PLANES == 2)
+                bf16Split(v[e], h[0], h[1]);
+            else
+                bf16Split3(v[e], h[0], h[1], h[2]);
+#pragma unroll
+            for (int pl = 0; pl < PLANES; ++pl)
+                pw[pl][e >> 1] |= (uint32_t)h[pl] << (16 * (e & 1));
         }
     }
-    uint4 *dst = frag + ((size_t)cb * nK + kb) * 2 * kWave + lane;
-    dst[0] = uint4{hw[0], hw[1], hw[2], hw[3]};
-    dst[kWave] = uint4{lw[0], lw[1], lw[2], lw[3]};
-}
-
-__global__ void buildFragF32Kernel(const float *B, int rows, int ld, int nColBlocks, int K, float4 *frag) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nK = K / 16;
-    if (idx >= (size_t)nColBlocks * nK * kWave)
-        return;
-    const int lane = (int)(idx % kWave);
-    const size_t rest = idx / kWave;
-    const int kb = (int)(rest % nK), cb = (int)(rest / nK);
-    const int col = cb * 16 + (lane & 15), k0 = kb * 16 + 4 * (lane >> 4);
-    frag[idx] = (col < rows) ? *reinterpret_cast<const float4 *>(B + (size_t)col * ld + k0) : float4{0.f, 0.f, 0.f, 0.f};
+    uint4 *dst = frag + ((size_t)cb * nK + kb) * PLANES * kWave + lane;
+#pragma unroll
+    for (int pl = 0; pl < PLANES; ++pl)
+        dst[pl * kWave] = uint4{pw[pl][0], pw[pl][1], pw[pl][2], pw[pl][3]};
 }
 
 void checkGemm(int nCols, int K, int ld, int Nw) {
@@ -537,7 +456,7 @@ void checkGemm(int nCols, int K, int ld, int Nw) {
 }  // namespace
 
 size_t fieldFragElemsBF16x2(int nCols, int K) { return 2 * (size_t)nCols * K; } /* hi and lo planes, uint16 each */
-size_t fieldFragElemsF32(int nCols, int K) { return (size_t)nCols * K; }
+size_t fieldFragElemsF32(int nCols, int K) { return 3 * (size_t)nCols * K; } /* hi, mid and lo planes */
 size_t fieldFragElemsFP8(int nCols, int K) { return (size_t)nCols * K; } /* bytes */
 
 void buildFragFP8(const uint8_t *d_B8, int rows, int ld8, int nCols, int K, uint8_t *d_frag, size_t fragElems,
@@ -589,20 +508,20 @@ void buildFragBF16x2(const float *d_B, int rows, int ld, int nCols, int K, uint1
     SQ_THROW_IF(fragElems < fieldFragElemsBF16x2(nCols, K), "bf16x2 fragment buffer holds %zu of %zu elements",
                 fragElems, fieldFragElemsBF16x2(nCols, K));
     const size_t threads = (size_t)(nCols / 16) * (K / 32) * kWave;
-    buildFragBF16x2Kernel<<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(
-        d_B, rows, ld, nCols / 16, K, reinterpret_cast<uint4 *>(d_frag));
+    buildFragBF16Kernel<2><<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(d_B, rows, ld, nCols / 16, K,
+                                                                               reinterpret_cast<uint4 *>(d_frag));
     HIP_CHECK(hipGetLastError());
 }
 
-void buildFragF32(const float *d_B, int rows, int ld, int nCols, int K, float *d_frag, size_t fragElems,
+void buildFragF32(const float *d_B, int rows, int ld, int nCols, int K, uint16_t *d_frag, size_t fragElems,
                   hipStream_t stream) {
     checkGemm(nCols, K, ld, K / 32);
     SQ_THROW_IF(rows > nCols, "field GEMM: %d rows exceed %d columns", rows, nCols);
     SQ_THROW_IF(fragElems < fieldFragElemsF32(nCols, K), "fp32 fragment buffer holds %zu of %zu elements", fragElems,
                 fieldFragElemsF32(nCols, K));
-    const size_t threads = (size_t)(nCols / 16) * (K / 16) * kWave;
-    buildFragF32Kernel<<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(
-        d_B, rows, ld, nCols / 16, K, reinterpret_cast<float4 *>(d_frag));
+    const size_t threads = (size_t)(nCols / 16) * (K / 32) * kWave;
+    buildFragBF16Kernel<3><<<(unsigned)((threads + 255) / 256), 256, 0, stream>>>(d_B, rows, ld, nCols / 16, K,
+                                                                               reinterpret_cast<uint4 *>(d_frag));
     HIP_CHECK(hipGetLastError());
 }
 
@@ -616,13 +535,13 @@ void fieldRefreshBF16x2(const uint16_t *d_frag, size_t fragElems, const uint32_t
     HIP_CHECK(hipGetLastError());
 }
 
-void fieldRefreshF32MFMA(const float *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
-                         int Nw, int ldF, float *d_F, hipStream_t stream) {
+void fieldRefreshF32MFMA(const uint16_t *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K,
+                         int m, int Nw, int ldF, float *d_F, hipStream_t stream) {
     checkGemm(nCols, K, K, Nw);
     SQ_THROW_IF(ldF < nCols, "field GEMM: output stride %d < %d columns", ldF, nCols);
     SQ_THROW_IF(fragElems < fieldFragElemsF32(nCols, K), "fp32 fragment buffer too small");
-    fieldRefreshF32Kernel<<<nCols / 16, kBlockThreads, 0, stream>>>(reinterpret_cast<const float4 *>(d_frag),
-                                                                    d_Qbits, K, m, Nw, ldF, d_F);
+    fieldRefreshBF16x3Kernel<<<nCols / 16, kBlockThreads, 0, stream>>>(reinterpret_cast<const uint4 *>(d_frag),
+                                                                       d_Qbits, K, m, Nw, ldF, d_F);
     HIP_CHECK(hipGetLastError());
 }
 

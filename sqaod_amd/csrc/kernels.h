@@ -110,18 +110,19 @@ void fieldRefresh(const real *d_B, int K, int nCols, int ldB, const uint32_t *d_
  * per row), fp32 accumulation, F row stride ldF.  B (rows x ld floats, zero
  * past its valid k) is first copied into MFMA fragment order:
  * fieldFragElemsBF16x2 = 2 nCols K uint16 (hi and lo planes) or
- * fieldFragElemsF32 = nCols K floats.  The launchers check the buffer sizes
+ * fieldFragElemsF32 = 3 nCols K uint16 (the fp32 form: B split exactly into
+ * three bf16 planes, all on the bf16 MFMA).  The launchers check the buffer sizes
  * they are given.  Dense annealer: B = J, rows = N, ld = nCols = K = Np. */
 size_t fieldFragElemsBF16x2(int nCols, int K);
 size_t fieldFragElemsF32(int nCols, int K);
 void buildFragBF16x2(const float *d_B, int rows, int ld, int nCols, int K, uint16_t *d_frag, size_t fragElems,
                      hipStream_t stream);
-void buildFragF32(const float *d_B, int rows, int ld, int nCols, int K, float *d_frag, size_t fragElems,
+void buildFragF32(const float *d_B, int rows, int ld, int nCols, int K, uint16_t *d_frag, size_t fragElems,
                   hipStream_t stream);
 void fieldRefreshBF16x2(const uint16_t *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
                         int Nw, int ldF, float *d_F, hipStream_t stream);
-void fieldRefreshF32MFMA(const float *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K, int m,
-                         int Nw, int ldF, float *d_F, hipStream_t stream);
+void fieldRefreshF32MFMA(const uint16_t *d_frag, size_t fragElems, const uint32_t *d_Qbits, int nCols, int K,
+                         int m, int Nw, int ldF, float *d_F, hipStream_t stream);
 /* FP8 form: B8 holds e4m3 bytes (rows x ld8) whose row c is scaled by
  * colScale[c] (a power of two); F[y][c] = colScale[c] sum_k q[y][k] B8[c][k]
  * on v_mfma_scale_f32_16x16x128_f8f6f4 (unit block scales), fp32

@@ -238,7 +238,7 @@ private:
     DevBuf<real> d_J_, d_h_, d_F_, d_U_;
     DevBuf<float> d_Jsplit_;  /* the form's J as floats: bf16x2 float(hi) + float(lo), fp8 J' */
     DevBuf<uint16_t> d_frag16_; /* bf16x2 MFMA refresh: J's hi/lo planes in fragment order */
-    DevBuf<float> d_frag32_;    /* fp32 MFMA refresh: J in fragment order */
+    DevBuf<uint16_t> d_frag32_; /* fp32 MFMA refresh: J's exact hi/mid/lo bf16 planes in fragment order */
     DevBuf<uint8_t> d_J8_;      /* fp8 form: e4m3 rows (N x ld8_), row i scaled by d_scale_[i] */
     DevBuf<uint8_t> d_frag8_;   /* fp8 MFMA refresh: d_J8_ in fragment order */
     DevBuf<float> d_scale_;     /* fp8 form: 2^a_i per row (Np, 1 past N) */
@@ -500,7 +500,7 @@ private:
 
     DevBuf<real> d_J_, d_JT_, d_h0_, d_h1_, d_dE_;
     DevBuf<uint16_t> d_fragJ16_, d_fragJT16_;
-    DevBuf<float> d_fragJ32_, d_fragJT32_;
+    DevBuf<uint16_t> d_fragJ32_, d_fragJT32_;
     DevBuf<uint32_t> d_Q0_, d_Q1_, d_words_;
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
