@@ -212,3 +212,21 @@ def test_large_instance_runs(rocm, oracle):
     assert np.allclose(E1, Eref, rtol=1e-5, atol=1e-2)
     cnt = an.get_counters()
     assert cnt["flips"] > 0 and cnt["batches"] > 0
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x2", "fp8"])
+def test_largest_lds_resident_n_bit_exact(oracle, rocm, form):
+    """N = 11520, the largest padded N whose float local-field sweep fits one
+    workgroup's LDS (fields, staged row, pending table, spin bits, neighbour
+    mirrors and windows: 12.4 bytes per site plus fixed parts, within 160 KiB),
+    m = 4: spins equal the serial CPU annealer's after three steps.  The row
+    DMA then spans 45 1-KiB ops (46 counted ops after the prefetch group) and
+    the pending table is 4096 entries for 11520 sites."""
+    N, m = 11520, 4
+    W = dense_int_W(N, SEED_BASE + 11)
+    ora, an = make_pair(oracle, rocm, W, m, seed=321, field_precision=form)
+    for G in schedule()[:3]:
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+        qa, qo = q_of(an), ora.get_q()
+        assert np.array_equal(qa, qo), f"{np.sum(qa != qo)} spins differ"
