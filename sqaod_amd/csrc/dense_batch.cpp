@@ -214,7 +214,7 @@ void GroupedDenseStepper::run(Device &dev, const std::vector<DenseGraphAnnealer<
             if (a.refreshDue()) {
                 FieldGemmJob j = a.takeFieldGemmJob();
                 gs.push_back(gBlocks);
-                gBlocks += j.nCols / 16;
+                gBlocks += fieldGemmJobBlocks(j, o.fieldFormat);
                 gj.push_back(j);
             }
             AttemptStreamJob s = a.takeAttemptStreamJob();

@@ -92,22 +92,43 @@ __device__ __forceinline__ void reduceStore(float (*red)[kTiles * 4][kWave], int
 
 typedef float RedTile[kTiles * 4][kWave];
 
+/* In a grouped launch the bf16 forms split the waves of a workgroup over
+ * columns instead of the k-range when the k-range is at most kColSplitMaxK:
+ * many small GEMMs then fill the device with a quarter of the workgroups and
+ * no LDS reduction, in the same summation order (refreshBF16Block).  A GEMM
+ * alone keeps the k-split, which fills the device better. */
+constexpr int kColSplitMaxK = 2048;
+__host__ __device__ inline bool bf16ColSplit(int K) { return K <= kColSplitMaxK; }
+
 /* frag: [column blocks][K/32 k-blocks][PLANES][64 lanes] x 16 B (PLANES bf16
  * planes whose sum is B: hi + lo for the bf16x2 form, hi + mid + lo -- exactly
  * B -- for the fp32 form); the block writes columns [16 cb, 16 cb + 16) of F
  * (row stride ldF).  lut: byte -> 8 spins as bf16 +-1, filled by the caller. */
 template <int PLANES>
 __device__ __forceinline__ void refreshBF16Block(const uint4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
-                                                 int ldF, float *F, int cb, RedTile *red, const uint4 *lut) {
+                                                 int ldF, float *F, int cbBlock, RedTile *red, const uint4 *lut,
+                                                 bool colSplit = false) {
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
     const int nK = K / 32;      /* k-blocks of 32; K is a multiple of 256 */
     const int nS = nK / kDepth; /* super-blocks */
+    /* Without colSplit the four waves take the super-blocks l = w, w + 4, ...
+     * of column block cbBlock and their partial tiles are added in LDS in the
+     * order ((P0 + P1) + P2) + P3.  With colSplit each wave owns a column block
+     * of its own (block cbBlock covers column blocks 4 cbBlock .. 4 cbBlock + 3)
+     * and forms the same partials P0..P3 one after the other, adding them in
+     * the same order: the results are bit-identical, without the LDS
+     * reduction and with a quarter of the workgroups. */
+    const int cb = colSplit ? cbBlock * kSplitK + w : cbBlock;
+    const int nParts = colSplit ? kSplitK : 1;
     const int sOff = (int)(((long long)cb * 3) % nS);
     const uint4 *fb = frag + (size_t)cb * nK * PLANES * kWave + lane;
     for (int y0 = 0; y0 < m; y0 += kRowsPerPass) {
-        f32x4 acc[kTiles];
+        f32x4 acc[kTiles], total[kTiles];
+      for (int part = 0; part < nParts; ++part) {
+        const int lStart = colSplit ? part : w;
+        constexpr int lStride = kSplitK;
 #pragma unroll
         for (int t = 0; t < kTiles; ++t)
             acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -179,17 +200,37 @@ __device__ __forceinline__ void refreshBF16Block(const uint4 *frag, const uint32
             }
         };
         /* buffers alternate with constant indices (no dynamic register indexing) */
-        for (int l = w; l < nS; l += 2 * kSplitK) {
-            if (l == w)
+        for (int l = lStart; l < nS; l += 2 * lStride) {
+            if (l == lStart)
                 fetch(0, l);
-            if (l + kSplitK < nS)
-                fetch(1, l + kSplitK);
+            if (l + lStride < nS)
+                fetch(1, l + lStride);
             compute(0);
-            if (l + kSplitK >= nS)
+            if (l + lStride >= nS)
                 break;
-            if (l + 2 * kSplitK < nS)
-                fetch(0, l + 2 * kSplitK);
+            if (l + 2 * lStride < nS)
+                fetch(0, l + 2 * lStride);
             compute(1);
+        }
+        if (colSplit) {
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    total[t][r] = part == 0 ? acc[t][r] : total[t][r] + acc[t][r];
+        }
+      }
+        if (colSplit) {
+            /* tile t, register r: row y0 + 16 t + 4 (lane / 16) + r, column 16 cb + lane % 16 */
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int y = y0 + 16 * t + 4 * g + r;
+                    if (y < m)
+                        F[(long long)y * ldF + cb * 16 + c] = total[t][r];
+                }
+            continue;
         }
 #pragma unroll
         for (int t = 0; t < kTiles; ++t)
@@ -376,7 +417,7 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshGroupedKernel(const
     if constexpr (FORM == fieldBF16x2) {
         fillLutBF16(lut);
         refreshBF16Block<2>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
-                            red, lut);
+                            red, lut, bf16ColSplit(jb.K));
     } else if constexpr (FORM == fieldFP8) {
         fillLutFP8(reinterpret_cast<uint2 *>(lut));
         refreshFP8Block(reinterpret_cast<const uint4 *>(jb.frag), jb.colScale, jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
@@ -384,7 +425,7 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshGroupedKernel(const
     } else {
         fillLutBF16(lut);
         refreshBF16Block<3>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
-                            red, lut);
+                            red, lut, bf16ColSplit(jb.K));
     }
 }
 
@@ -482,6 +523,10 @@ void fieldRefreshGrouped(int fieldFormat, const FieldGemmJob *d_jobs, const int 
     else
         fieldRefreshGroupedKernel<fieldF32><<<totalBlocks, kBlockThreads, 0, stream>>>(d_jobs, d_blockStart, nJobs);
     HIP_CHECK(hipGetLastError());
+}
+
+int fieldGemmJobBlocks(const FieldGemmJob &j, int fieldFormat) {
+    return j.nCols / (fieldFormat != fieldFP8 && bf16ColSplit(j.K) ? 16 * kSplitK : 16);
 }
 
 void checkFieldGemmJob(const FieldGemmJob &j, int fieldFormat) {

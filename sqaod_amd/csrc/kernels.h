@@ -137,9 +137,12 @@ struct FieldGemmJob {
     int nCols, K, m, Nw, ldF;
 };
 /* All jobs (one field format) in one launch: job j owns blocks
- * [blockStart[j], blockStart[j] + nCols_j / 16); totalBlocks is the sum.
+ * [blockStart[j], blockStart[j] + fieldGemmJobBlocks(job j)); totalBlocks is the sum.
  * checkFieldGemmJob validates a job's shape on the host. */
 void checkFieldGemmJob(const FieldGemmJob &job, int fieldFormat);
+/* workgroups of a job in a grouped launch (the bf16 forms give a short
+ * k-range four column blocks per workgroup) */
+int fieldGemmJobBlocks(const FieldGemmJob &job, int fieldFormat);
 void fieldRefreshGrouped(int fieldFormat, const FieldGemmJob *d_jobs, const int *d_blockStart, int nJobs,
                          int totalBlocks, hipStream_t stream);
 void buildFragFP8(const uint8_t *d_B8, int rows, int ld8, int nCols, int K, uint8_t *d_frag, size_t fragElems,
