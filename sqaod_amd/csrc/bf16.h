@@ -24,8 +24,9 @@ __device__ __forceinline__ void bf16Split(float v, uint16_t &hi, uint16_t &lo) {
 }
 
 /* v = hi + mid + lo exactly (three 8-bit significands cover fp32's 24; each
- * difference is exact, and the last remainder fits lo) for |v| >= 2^-110 or
- * v = 0: the fp32 form of the field GEMM runs on these three planes. */
+ * difference is exact, and the last remainder fits lo) with all three parts
+ * normal bf16 numbers when |v| >= 2^-103, and for v = 0: the fp32 form of the
+ * field GEMM runs on these three planes. */
 __device__ __forceinline__ void bf16Split3(float v, uint16_t &hi, uint16_t &mid, uint16_t &lo) {
     hi = bf16RoundNE(v);
     const float r1 = v - bf16ToFloat(hi);
