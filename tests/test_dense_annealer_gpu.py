@@ -67,10 +67,12 @@ def test_config1_bit_exact_trajectory(oracle, rocm, variant):
     assert np.array_equal(x, (ora.get_q() + 1) // 2)
 
 
-@pytest.mark.parametrize("N,m", [(100, 5), (64, 2), (37, 3), (200, 7), (256, 16)])
+@pytest.mark.parametrize("N,m", [(100, 5), (64, 2), (37, 3), (200, 7), (256, 16), (1, 1), (1, 3), (2, 1),
+                                 (33, 1)])
 def test_odd_shapes_bit_exact(oracle, rocm, N, m):
-    """odd m (ring neighbours 0 and m-1 share a colour), m = 2, N not a
-    multiple of 32: bit-exact over 30 steps."""
+    """odd m (ring neighbours 0 and m-1 share a colour), m = 2 (both
+    neighbours one slice), m = 1 (the slice is its own neighbour), N = 1 and N
+    not a multiple of 32: bit-exact over 30 steps."""
     W = dense_int_W(N, SEED_BASE + N + m)
     ora, an = make_pair(oracle, rocm, W, m, seed=7 + m)
     for G in schedule()[:30]:
