@@ -72,18 +72,11 @@ SQ_HD static inline float sq_expf_tab(float x, const uint64_t *tab) {
     const double C1 = 0x1.ebfce50fac4f3p-3 / (32.0 * 32.0);
     const double C2 = 0x1.62e42ff0c52d6p-1 / 32.0;
 
-    uint32_t ux = sq_f2u(x);
-    uint32_t abstop = (ux >> 20) & 0x7ff;
-    if (abstop >= (sq_f2u(88.0f) >> 20)) {
-        if (ux == 0xff800000u)                 /* -inf */
-            return 0.0f;
-        if (abstop >= (0x7f800000u >> 20))     /* +inf or nan */
-            return x + x;
-        if (x > 0x1.62e42ep6f)                 /* overflow: result rounds to +inf */
-            return 0x1p97f * 0x1p97f;
-        if (x < -0x1.9fe368p6f)                /* underflow: result rounds to +0 */
-            return 0x1p-95f * 0x1p-95f;
-    }
+    /* The special inputs are selected after the main path instead of
+     * branching before it (a wave whose lanes straddle |x| = 88 would run
+     * both sides); the main path is evaluated for every input, with a finite
+     * table index, and its result is replaced for x < -103.97 (rounds to +0,
+     * also -inf), x > 88.72 (rounds to +inf, also +inf) and NaN (x + x). */
     double xd = (double)x;
 #if SQ_EXPF_FUSED_REDUCTION
     double kd = __builtin_fma(InvLn2N, xd, Shift);
@@ -111,7 +104,11 @@ SQ_HD static inline float sq_expf_tab(float x, const uint64_t *tab) {
     y = p * r2 + y;
 #endif
     y = y * s;
-    return (float)y;
+    float res = (float)y;
+    res = (x < -0x1.9fe368p6f) ? 0x1p-95f * 0x1p-95f : res;
+    res = (x > 0x1.62e42ep6f) ? 0x1p97f * 0x1p97f : res;
+    res = (x != x) ? x + x : res;
+    return res;
 }
 
 SQ_HD static inline float sq_expf(float x) { return sq_expf_tab(x, sq_exp2_tab32); }
