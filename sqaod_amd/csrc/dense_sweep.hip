@@ -324,7 +324,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     const uint32_t tag = a.tag;
     int applied[2] = {0, 0};
     int k = 0;
-    unsigned long long nFlips = 0, nBatches = 0, nBlocked = 0;
+    uint32_t nFlips = 0, nBatches = 0, nBlocked = 0; /* per wave and sweep: at most N each */
     int spins = 0;
     bool aborted = false;
     /* kDefer: the J row of the last committed flip is loaded straight into
@@ -587,11 +587,14 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 /* CPUDenseGraphAnnealer.cpp:258-262, same operation order */
                 real dE = a.twoDivM * q * (hx + (real)2. * f);
                 dE -= q * ns_ * a.coef;
-                thr = (dE < (real)0.) ? (real)1. : Expo<real>::eval(-dE * a.beta, expTab);
+                /* evaluated for every lane and selected (no divergent branch) */
+                const real e = Expo<real>::eval(-dE * a.beta, expTab);
+                thr = (dE < (real)0.) ? (real)1. : e;
             } else {
                 /* :348-349 */
                 real dE = (real)2. * q * (hx + (real)2. * f);
-                thr = (dE < (real)0.) ? (real)1. : Expo<real>::eval(-dE * a.invKT, expTab);
+                const real e = Expo<real>::eval(-dE * a.invKT, expTab);
+                thr = (dE < (real)0.) ? (real)1. : e;
             }
             return thr > u;
         };
@@ -838,9 +841,9 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 a.Qbits[(long long)y * Nw + b + lane] = Qb[b + lane];
     }
     if (lane == 0 && a.counters) {
-        atomicAdd(&a.counters[0], nFlips);
-        atomicAdd(&a.counters[1], nBatches);
-        atomicAdd(&a.counters[2], nBlocked);
+        atomicAdd(&a.counters[0], (unsigned long long)nFlips);
+        atomicAdd(&a.counters[1], (unsigned long long)nBatches);
+        atomicAdd(&a.counters[2], (unsigned long long)nBlocked);
 #ifdef SQAOD_SWEEP_CYCLES
         for (int t = 0; t < 11; ++t)
             atomicAdd(&a.counters[3 + t], cyc[t]);
