@@ -497,10 +497,12 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 const bool keep1 = (nd[nb][1] >> 1) != tag || (nd[nb][1] & 1u);
                 pendSet[nb][0] = keep0 && e0 >= A && e0 < N && e0 - Kk < kWave - 1;
                 pendSet[nb][1] = keep1 && e1 >= A && e1 < N && e1 - Kk < kWave - 1;
-                if (pendSet[nb][0])
-                    pendShared[nb][0] = atomicMin(&tbl[ns[nb][0] & tblMask], pendCode(e0 - Kk, N, nb)) != kPendEmpty;
-                if (pendSet[nb][1])
-                    pendShared[nb][1] = atomicMin(&tbl[ns[nb][1] & tblMask], pendCode(e1 - Kk, N, nb)) != kPendEmpty;
+                /* every lane issues the atomic (no exec masking); the others
+                 * min with kPendEmpty, which leaves the slot as it is */
+                const int old0 = atomicMin(&tbl[ns[nb][0] & tblMask], pendSet[nb][0] ? pendCode(e0 - Kk, N, nb) : kPendEmpty);
+                const int old1 = atomicMin(&tbl[ns[nb][1] & tblMask], pendSet[nb][1] ? pendCode(e1 - Kk, N, nb) : kPendEmpty);
+                pendShared[nb][0] = pendSet[nb][0] && old0 != kPendEmpty;
+                pendShared[nb][1] = pendSet[nb][1] && old1 != kPendEmpty;
                 /* lane i needs entries [A, Kk + i): they must lie in the window */
                 const int winEnd = base + 2 * kWave < N ? base + 2 * kWave : N;
                 if (winEnd < N) {
@@ -676,10 +678,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         if (!a.sa) {
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
-                if (pendSetNow[nb][0])
-                    tbl[pendSite[nb][0] & tblMask] = kPendEmpty;
-                if (pendSetNow[nb][1])
-                    tbl[pendSite[nb][1] & tblMask] = kPendEmpty;
+                /* every lane clears its slot (no exec masking): every entry of
+                 * this batch is cleared, and a slot nobody entered is empty */
+                tbl[pendSite[nb][0] & tblMask] = kPendEmpty;
+                tbl[pendSite[nb][1] & tblMask] = kPendEmpty;
             }
         }
 
