@@ -406,8 +406,16 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         dmaEntries<4, true, kWin>(a.D + nN + base, winDL + nb * kWin * 4, base, N);
         dmaEntries<4, false, kWin>(a.X + nN + base, winSL + nb * kWin * 4, base, N);
     };
-    auto prefetch = [&]() {
-        ldsWait(); /* this batch's reads of the windows and the ring have returned */
+    /* waitReads: wait for this wave's outstanding LDS reads first.  The batch
+     * loop passes false: by then it has used every window entry it read (the
+     * ballots and the registration addresses), so those reads have returned,
+     * and the ring quarter requested is not one the batch reads (it reads
+     * quarters k/64 and k/64+1, the request is at most k/64+2 of 4).  Only the
+     * staged-row read of rowCorr may still be in flight, and no DMA here
+     * writes the staging buffer. */
+    auto prefetch = [&](bool waitReads) {
+        if (waitReads)
+            ldsWait();
         if (!a.sa) {
             loadNeighbour(0);
             loadNeighbour(1);
@@ -416,7 +424,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             requestQuarter();
         afterGroup = 0;
     };
-    prefetch();
+    prefetch(true);
     vmWaitImm<0>();
 
     /* load wait, neighbours, evaluate, commit, row, blocked; within neighbours:
@@ -551,7 +559,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         bool pendSetNow[2][2] = {{pendSet[0][0], pendSet[0][1]}, {pendSet[1][0], pendSet[1][1]}};
         uint32_t pendSite[2][2] = {{ns[0][0], ns[0][1]}, {ns[1][0], ns[1][1]}};
         SQ_CYC(n3);
-        prefetch();
+        prefetch(false);
         SQ_CYC(c2);
         SQ_CYC_ADD(cyc[1], c1, c2);
         SQ_CYC_ADD(cyc[6], c1, n1);
@@ -729,7 +737,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 aborted = true;
                 break;
             }
-            prefetch();
+            prefetch(true);
             waveSync();
             {
                 SQ_CYC(cb);
