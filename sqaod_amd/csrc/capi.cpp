@@ -431,6 +431,7 @@ int sqaod_rocm_expf_checksum(unsigned long long lo, unsigned long long hi, unsig
 
 /* ---------------- dense batch ---------------- */
 
+extern "C++" { /* internal helpers returning C++ references */
 namespace {
 DenseBatch &batchOf(sqaod_rocm_dense_batch *b) {
     SQ_THROW_IF(b == nullptr, "null batch handle");
@@ -443,6 +444,7 @@ DenseGraphAnnealer<float> &batchInstance(sqaod_rocm_dense_batch *b, int i, int m
     return a;
 }
 }  // namespace
+}  // extern "C++"
 
 int sqaod_rocm_dense_batch_new(sqaod_rocm_dense_batch **out) {
     return guarded([&] {
@@ -525,12 +527,14 @@ int sqaod_rocm_dense_batch_get_q(sqaod_rocm_dense_batch *b, int i, int8_t *q, in
 
 /* ---------------- serving ---------------- */
 
+extern "C++" { /* internal helper returning a C++ reference */
 namespace {
 Server &serverOf(sqaod_rocm_server *s) {
     SQ_THROW_IF(s == nullptr, "null server handle");
     return *reinterpret_cast<Server *>(s);
 }
 }  // namespace
+}  // extern "C++"
 
 int sqaod_rocm_server_new(sqaod_rocm_server **out) {
     return guarded([&] {
