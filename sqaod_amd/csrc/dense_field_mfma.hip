@@ -104,10 +104,9 @@ __host__ __device__ inline bool bf16ColSplit(int K) { return K <= kColSplitMaxK;
  * planes whose sum is B: hi + lo for the bf16x2 form, hi + mid + lo -- exactly
  * B -- for the fp32 form); the block writes columns [16 cb, 16 cb + 16) of F
  * (row stride ldF).  lut: byte -> 8 spins as bf16 +-1, filled by the caller. */
-template <int PLANES>
+template <int PLANES, bool colSplit = false>
 __device__ __forceinline__ void refreshBF16Block(const uint4 *frag, const uint32_t *Qbits, int K, int m, int Nw,
-                                                 int ldF, float *F, int cbBlock, RedTile *red, const uint4 *lut,
-                                                 bool colSplit = false) {
+                                                 int ldF, float *F, int cbBlock, RedTile *red, const uint4 *lut) {
     const int lane = threadIdx.x % kWave;
     const int w = threadIdx.x / kWave;
     const int c = lane & 15, g = lane >> 4;
@@ -416,16 +415,25 @@ __global__ __launch_bounds__(kBlockThreads) void fieldRefreshGroupedKernel(const
     const int cb = (int)blockIdx.x - blockStart[j];
     if constexpr (FORM == fieldBF16x2) {
         fillLutBF16(lut);
-        refreshBF16Block<2>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
-                            red, lut, bf16ColSplit(jb.K));
+        /* the split as a template argument: each path allocates its own registers */
+        if (bf16ColSplit(jb.K))
+            refreshBF16Block<2, true>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
+                                      jb.F, cb, red, lut);
+        else
+            refreshBF16Block<2, false>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
+                                       jb.F, cb, red, lut);
     } else if constexpr (FORM == fieldFP8) {
         fillLutFP8(reinterpret_cast<uint2 *>(lut));
         refreshFP8Block(reinterpret_cast<const uint4 *>(jb.frag), jb.colScale, jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
                         jb.F, cb, red, reinterpret_cast<const uint2 *>(lut));
     } else {
         fillLutBF16(lut);
-        refreshBF16Block<3>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF, jb.F, cb,
-                            red, lut, bf16ColSplit(jb.K));
+        if (bf16ColSplit(jb.K))
+            refreshBF16Block<3, true>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
+                                      jb.F, cb, red, lut);
+        else
+            refreshBF16Block<3, false>(reinterpret_cast<const uint4 *>(jb.frag), jb.Qbits, jb.K, jb.m, jb.Nw, jb.ldF,
+                                       jb.F, cb, red, lut);
     }
 }
 

@@ -2,10 +2,11 @@
 LDS address).  That is safe only while the compiler itself never uses M0 in
 the sweep kernels: this test compiles dense_sweep.hip for gfx950 with the
 library's flags and checks that every M0 reference in the device assembly is
-one of those inline-assembly M0 writes.  It also checks that no sweep kernel
-uses scratch memory: scratch accesses are vector-memory ops the sweep's
-explicit vmcnt waits do not count, which makes those waits wait for them, and
-each is a round trip to memory in the batch loop."""
+one of those inline-assembly M0 writes.  It also checks that no sweep or
+field-GEMM kernel uses scratch memory: scratch accesses are vector-memory ops
+the sweep's explicit vmcnt waits do not count, which makes those waits wait
+for them, and in either kernel each is a round trip to memory in the inner
+loop."""
 import os
 import re
 import shutil
@@ -23,25 +24,32 @@ def _hipcc():
     return None
 
 
-@pytest.fixture(scope="module")
-def sweep_asm(tmp_path_factory):
+def _device_asm(tmp_path_factory, src):
     hipcc = _hipcc()
     if hipcc is None:
         pytest.skip("hipcc not available")
     from sqaod_amd import build
-    out = tmp_path_factory.mktemp("asm") / "dense_sweep.s"
+    out = tmp_path_factory.mktemp("asm") / (src + ".s")
     cmd = [hipcc, *build.COMMON_FLAGS, f"-I{build.INCLUDE}", "-x", "hip", "--offload-device-only", "-S",
-           os.path.join(build.CSRC, "dense_sweep.hip"), "-o", str(out)]
+           os.path.join(build.CSRC, src), "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr
     return out.read_text()
 
 
-def test_sweep_kernels_use_no_scratch(sweep_asm):
-    """Every sweep kernel's code-object metadata reports a private segment of
-    0 bytes and no VGPR spills."""
-    kernels = re.findall(r"\.name:\s+(\S*denseSweep\S*)\n(.*?)(?=\n\s+- \.|\Z)", sweep_asm, flags=re.S)
-    assert kernels, "no sweep kernels in the metadata"
+@pytest.fixture(scope="module")
+def sweep_asm(tmp_path_factory):
+    return _device_asm(tmp_path_factory, "dense_sweep.hip")
+
+
+@pytest.mark.parametrize("src, pattern", [("dense_sweep.hip", "denseSweep"),
+                                          ("dense_field_mfma.hip", "fieldRefresh")])
+def test_kernels_use_no_scratch(tmp_path_factory, sweep_asm, src, pattern):
+    """Every sweep and field-GEMM kernel's code-object metadata reports a
+    private segment of 0 bytes and no VGPR spills."""
+    text = sweep_asm if src == "dense_sweep.hip" else _device_asm(tmp_path_factory, src)
+    kernels = re.findall(r"\.name:\s+(\S*" + pattern + r"\S*)\n(.*?)(?=\n\s+- \.|\Z)", text, flags=re.S)
+    assert kernels, f"no {pattern} kernels in the metadata"
     for name, meta in kernels:
         priv = re.search(r"\.private_segment_fixed_size:\s+(\d+)", meta)
         spill = re.search(r"\.vgpr_spill_count:\s+(\d+)", meta)
