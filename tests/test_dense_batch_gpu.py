@@ -70,6 +70,21 @@ def test_batch_field_forms(rocm, form):
         assert np.array_equal(b.get_q(i), q), i
 
 
+@pytest.mark.parametrize("form", ["fp32", "bf16x2"])
+def test_batch_k_split_gemm_path(rocm, form):
+    """An instance of 2500 sites (2560 padded, a k-range above 2048) takes the
+    grouped field GEMM's k-split path (the block's four waves share the
+    k-range and add their partial tiles in LDS) beside one that takes the
+    column split; both end equal to the instances annealed alone."""
+    steps = schedule()[:6]
+    probs = [(dense_normal_W(2500, SEED_BASE + 140), 4, 7001), (dense_int_W(300, SEED_BASE + 141), 3, 7002)]
+    b = batch(rocm, probs, steps, field_precision=form)
+    for i, (W, m, seed) in enumerate(probs):
+        q, E = alone(rocm, W, m, seed, steps, field_precision=form)
+        assert np.array_equal(b.get_q(i), q), f"instance {i} (N={W.shape[0]}, m={m})"
+        assert np.array_equal(b.get_E(i), E), i
+
+
 def test_autotune_and_schedule_split_do_not_change_results(rocm):
     """Autotuning restores spins, fields and streams; a schedule run as two
     graphs equals one run as a single graph."""
