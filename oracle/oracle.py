@@ -16,6 +16,7 @@ REF_RANDOM_PATH = os.path.join(_HERE, "_ref", "libsqaod_ref_random.so")
 
 ALGO_NAIVE, ALGO_COLORING, ALGO_SA_NAIVE, ALGO_SA_COLORING = 2, 3, 6, 7
 MINIMIZE, MAXIMIZE = 0, 1
+STREAM_MT, STREAM_PHILOX = 0, 1
 
 _lib = None
 
@@ -82,6 +83,16 @@ def lib():
             "sqo_dg_bf_search": (i32, [i32, i32, vp, i32, i32, vp, vp]),
             "sqo_bg_bf_search": (i32, [i32, i32, i32, vp, vp, vp, i32, i32, i32, vp, vp, vp]),
             "sqo_expf_checksum": (u64, [u64, u64]),
+            "sqo_philox4x32_10": (None, [vp, vp, vp]),
+            "sqo_philox_attempts": (None, [u64, u64, i32, i32, vp, vp, i32]),
+            "sqo_philox_spins": (None, [u64, ctypes.c_uint32, i32, i32, vp, sz, i32]),
+            "sqo_philox_naive": (None, [u64, u64, u64, vp]),
+            "sqo_philox_bg": (None, [u64, u64, i32, u64, vp]),
+            "sqo_dg_set_stream": (None, [vp, i32]),
+            "sqo_dg_set_local_fields": (None, [vp, i32]),
+            "sqo_dg_anneal_one_step_injected": (None, [vp, f64, f64, vp, vp, vp]),
+            "sqo_bg_set_stream": (None, [vp, i32]),
+            "sqo_bg_set_local_fields": (None, [vp, i32]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -104,6 +115,29 @@ def mt_stream(seed, n, skip=0):
     out = np.empty(n, np.uint32)
     lib().sqo_mt_stream(seed, skip, _p(out), n)
     return out
+
+
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block (Salmon et al., SC'11)."""
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().sqo_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def philox_attempts(seed, sweep, N, m, dtype=np.float32):
+    """The rocm device's attempt arrays (X, U) of one dense sweep."""
+    X = np.empty((m, N), np.uint32)
+    U = np.empty((m, N), dtype)
+    lib().sqo_philox_attempts(seed, sweep, N, m, _p(X), _p(U), _dt(dtype))
+    return X, U
+
+
+def philox_spins(seed, salt, N, m):
+    q = np.empty((m, N), np.float64)
+    lib().sqo_philox_spins(seed, salt, N, m, _p(q), 8, N)
+    return q.astype(np.int8)
 
 
 def expf_checksum(lo=0, hi=1 << 32):
@@ -225,9 +259,31 @@ class DenseGraphAnnealer:
     def last_flips(self):
         return lib().sqo_dg_last_flips(self._h)
 
+    def set_stream(self, kind):
+        """STREAM_MT (sqaod.cpu's per-worker MT19937) or STREAM_PHILOX (the
+        rocm device's counter layout)."""
+        lib().sqo_dg_set_stream(self._h, kind)
+
+    def set_local_fields(self, on=True):
+        """Incremental local fields instead of the per-attempt dot (the same
+        results for integer-valued problems)."""
+        lib().sqo_dg_set_local_fields(self._h, 1 if on else 0)
+
+    def anneal_one_step_injected(self, G, beta, X, U, Y=None):
+        """One step consuming the given stream: X, U of shape (m, N) in (slice,
+        attempt) order; for the naive algorithm Y, X, U of N m entries."""
+        X = np.ascontiguousarray(X, np.uint32)
+        U = np.ascontiguousarray(U, self.dtype)
+        Yp = None
+        if Y is not None:
+            Y = np.ascontiguousarray(Y, np.uint32)
+            Yp = _p(Y)
+        lib().sqo_dg_anneal_one_step_injected(self._h, float(G), float(beta), Yp, _p(X), _p(U))
+
 
 class BipartiteGraphAnnealer:
-    """Oracle counterpart of sqaod.cpu.BipartiteGraphAnnealer (one worker)."""
+    """Oracle counterpart of sqaod.cpu.BipartiteGraphAnnealer (serial when
+    n_workers == 1, the OpenMP static-chunk variant otherwise)."""
 
     def __init__(self, dtype=np.float32, n_workers=1):
         self.dtype = np.dtype(dtype)
@@ -301,6 +357,12 @@ class BipartiteGraphAnnealer:
 
     def get_system_E(self, G, beta):
         return lib().sqo_bg_get_system_E(self._h, float(G), float(beta))
+
+    def set_stream(self, kind):
+        lib().sqo_bg_set_stream(self._h, kind)
+
+    def set_local_fields(self, on=True):
+        lib().sqo_bg_set_local_fields(self._h, 1 if on else 0)
 
 
 def bipartite_hamiltonian(b0, b1, W):

@@ -68,6 +68,13 @@ double sqo_mt_f64(sqo_mt *r) {
     return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
 }
 
+/* Random.cpp:157-168 conversions of raw words */
+float sqo_word_to_f32(uint32_t w) { return (float)w * (float)(1. / 4294967296.); }
+double sqo_words_to_f64(uint32_t w1, uint32_t w2) {
+    uint32_t a = w1 >> 5, b = w2 >> 6;
+    return ((double)a * 67108864.0 + (double)b) * (1.0 / 9007199254740992.0);
+}
+
 void sqo_mt_stream(unsigned long long seed, unsigned long long skip, uint32_t *out, size_t n) {
     sqo_mt r;
     sqo_mt_seed(&r, seed);
@@ -75,6 +82,155 @@ void sqo_mt_stream(unsigned long long seed, unsigned long long skip, uint32_t *o
         (void)sqo_mt_u32(&r);
     for (size_t i = 0; i < n; ++i)
         out[i] = sqo_mt_u32(&r);
+}
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:  */
+/* as easy as 1, 2, 3", SC'11): the counter-based stream of the rocm     */
+/* device, restated with the device's counter layouts so that the CPU    */
+/* annealers can consume the same stream.                               */
+/* ------------------------------------------------------------------ */
+void sqo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    /* ten rounds of two 32x32->64 multiplications (multipliers 0xD2511F53,
+     * 0xCD9E8D57) with the Weyl key schedule (0x9E3779B9, 0xBB67AE85) */
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* dense attempt (y, k) of sweep s: counter (k, y, s_lo, s_hi), key = seed */
+void sqo_philox_attempt(unsigned long long seed, unsigned long long sweep, uint32_t k, uint32_t y, uint32_t out[4]) {
+    const uint32_t c[4] = {k, y, (uint32_t)sweep, (uint32_t)(sweep >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    sqo_philox4x32_10(c, key, out);
+}
+
+/* naive attempt t of sweep s: counter (t_lo, t_hi, s_lo, s_hi ^ 0x20000000) */
+void sqo_philox_naive(unsigned long long seed, unsigned long long sweep, unsigned long long t, uint32_t out[4]) {
+    const uint32_t c[4] = {(uint32_t)t, (uint32_t)(t >> 32), (uint32_t)sweep,
+                           (uint32_t)(sweep >> 32) ^ 0x20000000u};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    sqo_philox4x32_10(c, key, out);
+}
+
+/* bipartite half-step of step index s, side 1 (q1 annealed) or 0:
+ * counter (pos_lo, pos_hi, s_lo, s_hi ^ (side ? 0x80000000 : 0x40000000)) */
+void sqo_philox_bg(unsigned long long seed, unsigned long long step, int side, unsigned long long pos,
+                   uint32_t out[4]) {
+    const uint32_t c[4] = {(uint32_t)pos, (uint32_t)(pos >> 32), (uint32_t)step,
+                           (uint32_t)(step >> 32) ^ (side ? 0x80000000u : 0x40000000u)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    sqo_philox4x32_10(c, key, out);
+}
+
+/* spins of an m x N block (row stride ld, elements of esize bytes, float or
+ * double): q[y][x] = +1 if bit 0 of word x % 4 of the block at counter
+ * (x / 4, y, salt, 0xffffffff) is set, else -1 */
+void sqo_philox_spins(unsigned long long seed, uint32_t salt, int N, int m, void *Q, size_t esize, int ld) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int y = 0; y < m; ++y)
+        for (int x0 = 0; x0 < N; x0 += 4) {
+            const uint32_t c[4] = {(uint32_t)(x0 / 4), (uint32_t)y, salt, 0xffffffffu};
+            uint32_t r[4];
+            sqo_philox4x32_10(c, key, r);
+            for (int b = 0; b < 4 && x0 + b < N; ++b) {
+                double v = (r[b] & 1u) ? 1. : -1.;
+                size_t i = (size_t)y * ld + x0 + b;
+                if (esize == 4)
+                    ((float *)Q)[i] = (float)v;
+                else
+                    ((double *)Q)[i] = v;
+            }
+        }
+}
+
+void sqo_philox_attempts(unsigned long long seed, unsigned long long sweep, int N, int m, uint32_t *X, void *U,
+                         int dtype) {
+    /* the device's per-sweep attempt arrays: X[y][k] = r0 % N, U[y][k] from
+     * r1 (float) or (r1, r2) (double) */
+    for (int y = 0; y < m; ++y)
+        for (int k = 0; k < N; ++k) {
+            uint32_t r[4];
+            sqo_philox_attempt(seed, sweep, (uint32_t)k, (uint32_t)y, r);
+            size_t i = (size_t)y * N + k;
+            X[i] = r[0] % (uint32_t)N;
+            if (dtype)
+                ((double *)U)[i] = sqo_words_to_f64(r[1], r[2]);
+            else
+                ((float *)U)[i] = sqo_word_to_f32(r[1]);
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* dot accumulated in double, 16 lanes in a fixed order (the bipartite   */
+/* half-step GEMM of the oracle; exact for integer data)                */
+/* ------------------------------------------------------------------ */
+#if defined(__x86_64__)
+#include <immintrin.h>
+__attribute__((target("avx2"))) static double dot64_f32_avx2(const float *a, const float *b, int n) {
+    __m256d s[4] = {_mm256_setzero_pd(), _mm256_setzero_pd(), _mm256_setzero_pd(), _mm256_setzero_pd()};
+    int i = 0;
+    for (; i + 16 <= n; i += 16)
+        for (int l = 0; l < 4; ++l)
+            s[l] = _mm256_add_pd(s[l], _mm256_mul_pd(_mm256_cvtps_pd(_mm_loadu_ps(a + i + 4 * l)),
+                                                     _mm256_cvtps_pd(_mm_loadu_ps(b + i + 4 * l))));
+    double lane[16];
+    for (int l = 0; l < 4; ++l)
+        _mm256_storeu_pd(lane + 4 * l, s[l]);
+    for (; i < n; ++i)
+        lane[i & 15] += (double)a[i] * (double)b[i];
+    double r = 0.;
+    for (int l = 0; l < 16; ++l)
+        r += lane[l];
+    return r;
+}
+#endif
+static double dot64_f32_plain(const float *a, const float *b, int n) {
+    double lane[16] = {0.};
+    int i = 0;
+    for (; i + 16 <= n; i += 16)
+        for (int l = 0; l < 16; ++l)
+            lane[l] += (double)a[i + l] * (double)b[i + l];
+    for (; i < n; ++i)
+        lane[i & 15] += (double)a[i] * (double)b[i];
+    double r = 0.;
+    for (int l = 0; l < 16; ++l)
+        r += lane[l];
+    return r;
+}
+static double dot64_f32(const float *a, const float *b, int n) {
+#if defined(__x86_64__)
+    static int avx2 = -1;
+    if (avx2 < 0)
+        avx2 = __builtin_cpu_supports("avx2") ? 1 : 0;
+    if (avx2)
+        return dot64_f32_avx2(a, b, n);
+#endif
+    return dot64_f32_plain(a, b, n);
+}
+static double dot64_f64(const double *a, const double *b, int n) {
+    double lane[16] = {0.};
+    int i = 0;
+    for (; i + 16 <= n; i += 16)
+        for (int l = 0; l < 16; ++l)
+            lane[l] += a[i + l] * b[i + l];
+    for (; i < n; ++i)
+        lane[i & 15] += a[i] * b[i];
+    double r = 0.;
+    for (int l = 0; l < 16; ++l)
+        r += lane[l];
+    return r;
 }
 
 /* ------------------------------------------------------------------ */
@@ -177,6 +333,8 @@ void sqo_dg_energy_x_f64acc(int N, int nx, const double *W, const int8_t *x, dou
 #define LOGR(v) logf(v)
 #define DOTR(a, b, n) sqo_dot_f32(a, b, n)
 #define RANDR(r) sqo_mt_f32(r)
+#define PHILOX_U0(p) sqo_word_to_f32((p)[0])
+#define DOT64(a, b, n) dot64_f32(a, b, n)
 #define PAD 16
 #include "oracle_typed.inc"
 #undef REAL
@@ -186,6 +344,8 @@ void sqo_dg_energy_x_f64acc(int N, int nx, const double *W, const int8_t *x, dou
 #undef LOGR
 #undef DOTR
 #undef RANDR
+#undef PHILOX_U0
+#undef DOT64
 #undef PAD
 
 #define REAL double
@@ -195,6 +355,8 @@ void sqo_dg_energy_x_f64acc(int N, int nx, const double *W, const int8_t *x, dou
 #define LOGR(v) log(v)
 #define DOTR(a, b, n) sqo_dot_f64(a, b, n)
 #define RANDR(r) sqo_mt_f64(r)
+#define PHILOX_U0(p) sqo_words_to_f64((p)[0], (p)[1])
+#define DOT64(a, b, n) dot64_f64(a, b, n)
 #define PAD 8
 #include "oracle_typed.inc"
 #undef REAL
@@ -204,6 +366,8 @@ void sqo_dg_energy_x_f64acc(int N, int nx, const double *W, const int8_t *x, dou
 #undef LOGR
 #undef DOTR
 #undef RANDR
+#undef PHILOX_U0
+#undef DOT64
 #undef PAD
 
 /* ------------------------------------------------------------------ */
@@ -266,6 +430,13 @@ double sqo_dg_get_system_E(sqo_dg *a, double G, double beta) {
     return a->dtype ? dg_get_system_E_f64(a->impl, G, beta)
                     : (double)dg_get_system_E_f32(a->impl, (float)G, (float)beta);
 }
+void sqo_dg_set_stream(sqo_dg *a, int kind) { DG_DISPATCH(dg_set_stream_f32(a->impl, kind), dg_set_stream_f64(a->impl, kind)); }
+void sqo_dg_set_local_fields(sqo_dg *a, int on) { DG_DISPATCH(dg_set_local_f32(a->impl, on), dg_set_local_f64(a->impl, on)); }
+void sqo_dg_anneal_one_step_injected(sqo_dg *a, double G, double beta, const uint32_t *Y, const uint32_t *X,
+                                     const void *U) {
+    DG_DISPATCH(dg_anneal_one_step_injected_f32(a->impl, (float)G, (float)beta, Y, X, (const float *)U),
+                dg_anneal_one_step_injected_f64(a->impl, G, beta, Y, X, (const double *)U));
+}
 long long sqo_dg_last_flips(const sqo_dg *a) {
     return a->dtype ? dg_last_flips_f64(a->impl) : dg_last_flips_f32(a->impl);
 }
@@ -325,6 +496,8 @@ void sqo_bg_anneal_one_step(sqo_bg *a, double G, double beta) {
 void sqo_bg_calculate_E(sqo_bg *a, void *E) {
     DG_DISPATCH(bg_calculate_E_f32(a->impl, (float *)E), bg_calculate_E_f64(a->impl, (double *)E));
 }
+void sqo_bg_set_stream(sqo_bg *a, int kind) { DG_DISPATCH(bg_set_stream_f32(a->impl, kind), bg_set_stream_f64(a->impl, kind)); }
+void sqo_bg_set_local_fields(sqo_bg *a, int on) { DG_DISPATCH(bg_set_local_f32(a->impl, on), bg_set_local_f64(a->impl, on)); }
 double sqo_bg_get_system_E(sqo_bg *a, double G, double beta) {
     return a->dtype ? bg_get_system_E_f64(a->impl, G, beta)
                     : (double)bg_get_system_E_f32(a->impl, (float)G, (float)beta);

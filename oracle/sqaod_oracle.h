@@ -50,6 +50,20 @@ double sqo_mt_f64(sqo_mt *r);                              /* Random.cpp:165-168
 /* n consecutive tempered outputs after skipping `skip` outputs. */
 void sqo_mt_stream(unsigned long long seed, unsigned long long skip, uint32_t *out, size_t n);
 
+float sqo_word_to_f32(uint32_t w);                          /* Random.cpp:157-161 */
+double sqo_words_to_f64(uint32_t w1, uint32_t w2);          /* Random.cpp:165-168 */
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) in the rocm device's layouts ---- */
+void sqo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void sqo_philox_attempt(unsigned long long seed, unsigned long long sweep, uint32_t k, uint32_t y, uint32_t out[4]);
+void sqo_philox_naive(unsigned long long seed, unsigned long long sweep, unsigned long long t, uint32_t out[4]);
+void sqo_philox_bg(unsigned long long seed, unsigned long long step, int side, unsigned long long pos,
+                   uint32_t out[4]);
+void sqo_philox_spins(unsigned long long seed, uint32_t salt, int N, int m, void *Q, size_t esize, int ld);
+/* X[y][k], U[y][k] (dtype 0 float, 1 double) of one dense sweep */
+void sqo_philox_attempts(unsigned long long seed, unsigned long long sweep, int N, int m, uint32_t *X, void *U,
+                         int dtype);
+
 /* ---- SIMD dot order (sqaodc/cpu/Dot_SIMD.cpp:15-110) ---- */
 float sqo_dot_f32(const float *v0, const float *v1, int n);    /* dot_sse2(float), :60-110 */
 double sqo_dot_f64(const double *v0, const double *v1, int n); /* dot_sse2(double), :15-55 */
@@ -67,6 +81,9 @@ void sqo_dg_energy_x_f64acc(int N, int nx, const double *W, const int8_t *x, dou
 /* ---- dense annealer (sqaodc/cpu/CPUDenseGraphAnnealer.cpp) ---- */
 enum { SQO_ALGO_NAIVE = 2, SQO_ALGO_COLORING = 3, SQO_ALGO_SA_NAIVE = 6 }; /* Preference.h:8-17 */
 enum { SQO_MINIMIZE = 0, SQO_MAXIMIZE = 1 };                                /* Solver.h:9-14 */
+/* attempt streams: sqaod.cpu's per-worker MT19937 (default), the rocm
+ * device's Philox layout, or arrays injected per step */
+enum { SQO_STREAM_MT = 0, SQO_STREAM_PHILOX = 1, SQO_STREAM_INJECTED = 2 };
 
 typedef struct sqo_dg sqo_dg;
 /* dtype: 0 = float, 1 = double.  nWorkers: size of the per-worker RNG array
@@ -89,6 +106,14 @@ void sqo_dg_calculate_E(sqo_dg *a, void *E);                          /* :206-21
 double sqo_dg_get_system_E(sqo_dg *a, double G, double beta);         /* :230-247 */
 /* number of accepted flips in the last anneal_one_step (diagnostic) */
 long long sqo_dg_last_flips(const sqo_dg *a);
+void sqo_dg_set_stream(sqo_dg *a, int kind);
+/* incremental local fields instead of the per-attempt dot: the same results
+ * for integer-valued problems (every sum exact), much faster at large N */
+void sqo_dg_set_local_fields(sqo_dg *a, int on);
+/* one step consuming the caller's stream: coloring/SA take X, U of m x N in
+ * (slice, attempt) order; naive takes Y, X, U of N m entries in serial order */
+void sqo_dg_anneal_one_step_injected(sqo_dg *a, double G, double beta, const uint32_t *Y, const uint32_t *X,
+                                     const void *U);
 
 /* ---- bipartite annealer (sqaodc/cpu/CPUBipartiteGraphAnnealer.cpp) ---- */
 typedef struct sqo_bg sqo_bg;
@@ -111,6 +136,8 @@ void sqo_bg_get_q(const sqo_bg *a, int8_t *q0, int8_t *q1);
 void sqo_bg_anneal_one_step(sqo_bg *a, double G, double beta);        /* :288-542 */
 void sqo_bg_calculate_E(sqo_bg *a, void *E);                          /* :215-222 */
 double sqo_bg_get_system_E(sqo_bg *a, double G, double beta);         /* :268-286 */
+void sqo_bg_set_stream(sqo_bg *a, int kind);                          /* MT (per worker) or Philox */
+void sqo_bg_set_local_fields(sqo_bg *a, int on);                      /* incremental dEmat (integer data) */
 void sqo_bg_hamiltonian_f64(int N0, int N1, const double *b0, const double *b1,
                             const double *W, double *h0, double *h1, double *J, double *c);
 void sqo_bg_energy_q_f64acc(int N0, int N1, int m, const double *h0, const double *h1,
