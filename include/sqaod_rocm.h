@@ -43,9 +43,13 @@ typedef struct sqaod_rocm_server sqaod_rocm_server;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *sqaod_rocm_last_error(void);
-/* Library version; replaces sqaodc_cuda_version (sqaodc/cuda/api_cuda.cpp:5-9,
- * probed by sqaodpy/sqaod/common/envcheck.py:15-62). */
-void sqaod_rocm_version(int *major, int *minor);
+/* Version probe; replaces sqaodc_cuda_version(int *version, int *cuda_version)
+ * (sqaodc/cuda/api_cuda.cpp:5-9), found by name the way
+ * sqaodpy/sqaod/common/envcheck.py:27-37 probes the CUDA library:
+ * *version = the sqaodc API version (10003 = 1.0.3, the SQAODC_VERSION
+ * encoding of sqaodc/common/defines.h:80), *hip_version = HIP_VERSION of the
+ * HIP the library was built with (major * 10^7 + minor * 10^5 + patch). */
+void sqaodc_rocm_version(int *version, int *hip_version);
 /* Number of visible GPUs (sqaod.is_cuda_available counterpart, Common.cpp:20-77). */
 int sqaod_rocm_device_count(int *count);
 
@@ -78,7 +82,10 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
  * "slices_per_block" (int, 0 = auto), "sweep_launch" ("auto" | "cooperative"),
  * "refresh_interval" (int, steps between
  * field recomputes; 0 = only after the spins were set), "refresh_engine"
- * ("mfma" | "vector"), "profile" ("0" | "1"). */
+ * ("mfma" | "vector"), "profile" ("0" | "1"), "spin_limit" (int > 0: blocked
+ * iterations a slice may wait on a neighbour's decision before the sweep gives
+ * up; the step then fails with an error instead of hanging, and the spins must
+ * be set again). */
 int sqaod_rocm_dg_annealer_set_preference(sqaod_rocm_dg_annealer *a, const char *name, const char *value);
 /* annealer_get_preferences (:302-329): "name=value" pairs joined by ';'.
  * Writes at most buflen bytes (NUL-terminated); *needed receives the full length + 1. */
@@ -334,9 +341,12 @@ int sqaod_rocm_dense_batch_get_q(sqaod_rocm_dense_batch *b, int i, int8_t *q, in
 int sqaod_rocm_server_new(sqaod_rocm_server **out);
 int sqaod_rocm_server_delete(sqaod_rocm_server *s);
 /* memory_budget_mb, max_active, dense_field_precision (fp32 | bf16x2 | fp8),
- * bipartite_field_precision (fp32 | bf16x2), bipartite_streams, block_fraction */
+ * bipartite_field_precision (fp32 | bf16x2), bipartite_streams, block_fraction,
+ * dense_spin_limit (the bounded wait of dense jobs admitted afterwards; 0 = default) */
 int sqaod_rocm_server_set_preference(sqaod_rocm_server *s, const char *name, const char *value);
-/* W: N x N symmetric (dense) or N1 x N0 (bipartite, b0[N0], b1[N1]); minimized */
+/* W: N x N symmetric (dense) or N1 x N0 (bipartite, b0[N0], b1[N1]); minimized.
+ * A job over the memory budget, or a dense job whose N or m the grouped sweep
+ * cannot hold (LDS region per slice, resident waves), is rejected here. */
 int sqaod_rocm_server_submit_dense(sqaod_rocm_server *s, const float *W, int N, int m, unsigned long long seed,
                                    const double *G, const double *beta, int nSteps, int *id);
 int sqaod_rocm_server_submit_bipartite(sqaod_rocm_server *s, const float *b0, const float *b1, const float *W, int N0,
@@ -351,9 +361,14 @@ int sqaod_rocm_server_status(sqaod_rocm_server *s, int *pending, int *active, un
 int sqaod_rocm_server_take_done(sqaod_rocm_server *s, int *ids, int cap, int *n);
 int sqaod_rocm_server_result_shape(sqaod_rocm_server *s, int id, int *kind, int *N0, int *N1, int *m);
 /* E[m]; x0[m x N0] (dense: x), x1[m x N1] (bipartite, else may be null);
- * times: submit, admit, finish (seconds on the server clock); attempts of the job */
+ * times: submit, admit, finish (seconds on the server clock); attempts of the job.
+ * Fails (-1, the job's error as the message) for a failed job. */
 int sqaod_rocm_server_result(sqaod_rocm_server *s, int id, float *E, int8_t *x0, int8_t *x1, double times[3],
                              long long *attempts);
+/* *failed = 1 when the job failed -- its admission, a step or its solution
+ * threw (a HIP error, an aborted sweep) -- with the message in buf (at most
+ * buflen bytes, NUL-terminated); the server goes on with the other jobs. */
+int sqaod_rocm_server_result_error(sqaod_rocm_server *s, int id, char *buf, size_t buflen, int *failed);
 int sqaod_rocm_server_release(sqaod_rocm_server *s, int id);
 int sqaod_rocm_server_now(sqaod_rocm_server *s, double *t);
 

@@ -10,6 +10,8 @@
 #include <type_traits>
 
 #include "sqaod_rocm.h"
+#include <hip/hip_version.h>
+
 #include "solver.h"
 
 using namespace sqaod_rocm;
@@ -152,11 +154,11 @@ extern "C" {
 
 const char *sqaod_rocm_last_error(void) { return g_lastError.c_str(); }
 
-void sqaod_rocm_version(int *major, int *minor) {
-    if (major)
-        *major = 1;
-    if (minor)
-        *minor = 0;
+void sqaodc_rocm_version(int *version, int *hip_version) {
+    if (version)
+        *version = 10003; /* the sqaodc solver API this library implements (sqaodc 1.0.3) */
+    if (hip_version)
+        *hip_version = HIP_VERSION; /* major * 10^7 + minor * 10^5 + patch of the HIP it was built with */
 }
 
 int sqaod_rocm_device_count(int *count) {
@@ -618,6 +620,7 @@ int sqaod_rocm_server_result(sqaod_rocm_server *s, int id, float *E, int8_t *x0,
                              long long *attempts) {
     return guarded([&] {
         const Server::Result &r = serverOf(s).result(id);
+        SQ_THROW_IF(!r.error.empty(), "job %d failed: %s", id, r.error.c_str());
         if (E)
             std::copy(r.E.begin(), r.E.end(), E);
         if (x0)
@@ -631,6 +634,18 @@ int sqaod_rocm_server_result(sqaod_rocm_server *s, int id, float *E, int8_t *x0,
         }
         if (attempts)
             *attempts = r.attempts;
+    });
+}
+
+int sqaod_rocm_server_result_error(sqaod_rocm_server *s, int id, char *buf, size_t buflen, int *failed) {
+    return guarded([&] {
+        const Server::Result &r = serverOf(s).result(id);
+        *failed = r.error.empty() ? 0 : 1;
+        if (buf && buflen) {
+            const size_t n = std::min(buflen - 1, r.error.size());
+            memcpy(buf, r.error.data(), n);
+            buf[n] = 0;
+        }
     });
 }
 

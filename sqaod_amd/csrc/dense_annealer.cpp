@@ -355,7 +355,9 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
         else
             profile_ = atoi(value.c_str()) < 0 ? 0 : atoi(value.c_str());
     } else if (name == "spin_limit") {
-        spinLimit_ = atoi(value.c_str());
+        const int v = atoi(value.c_str());
+        SQ_THROW_IF(v <= 0, "spin_limit must be positive");
+        spinLimit_ = v;
     } else if (name == "precision" || name == "device") {
         /* read-only (Preference.h:26-36) */
     } else if (name == "experiment" || name == "tile_size") {

@@ -150,6 +150,7 @@ void DenseBatch::run(const double *G, const double *beta, int nSteps, bool graph
 
 void GroupedDenseStepper::run(Device &dev, const std::vector<DenseGraphAnnealer<float> *> &inst,
                               const std::vector<int> &order, int nSteps, const Schedule &sched, const Options &o) {
+    lastAborted_.clear();
     SQ_THROW_IF(inst.empty() || order.empty(), "grouped step: no instances");
     SQ_THROW_IF(nSteps <= 0, "no steps");
     dev.makeCurrent();
@@ -322,8 +323,17 @@ void GroupedDenseStepper::run(Device &dev, const std::vector<DenseGraphAnnealer<
     if (!o.wait)
         HIP_CHECK(hipStreamSynchronize(st)); /* the plan buffer and graph are released below */
     cleanup();
-    for (int i : order)
-        inst[i]->checkStepAbort();
+    std::string first;
+    for (int i : order) {
+        try {
+            inst[i]->checkStepAbort();
+        } catch (const std::exception &e) {
+            lastAborted_.push_back(i);
+            if (first.empty())
+                first = e.what();
+        }
+    }
+    SQ_THROW_IF(!lastAborted_.empty(), "%s (%zu instance(s))", first.c_str(), lastAborted_.size());
 }
 
 void DenseBatch::autotune(double G, double beta, int trialSteps) {

@@ -36,6 +36,7 @@ struct Server::Job {
     std::vector<double> G, beta;
     int step = 0;
     size_t bytes = 0;
+    bool admitted = false, failed = false;
     double submitS = 0., admitS = 0.;
     std::unique_ptr<DenseGraphAnnealer<float>> dense;
     std::unique_ptr<BipartiteGraphAnnealer<float>> bip;
@@ -108,6 +109,11 @@ void Server::setPreference(const std::string &name, const std::string &value) {
     } else if (name == "block_fraction") {
         blockFraction_ = atof(value.c_str());
         SQ_THROW_IF(!(blockFraction_ > 0. && blockFraction_ <= 1.), "block_fraction must be in (0, 1]");
+    } else if (name == "dense_spin_limit") {
+        /* the sweep's bounded wait of the dense jobs admitted from now on
+         * (0: the annealer's default); a tiny limit makes their steps fail */
+        denseSpinLimit_ = atoi(value.c_str());
+        SQ_THROW_IF(denseSpinLimit_ < 0, "dense_spin_limit must be >= 0");
     } else {
         SQ_THROW("unknown server preference '%s'", name.c_str());
     }
@@ -155,6 +161,13 @@ int Server::submitDense(const float *W, int N, int m, unsigned long long seed, c
     j->beta.assign(beta, beta + nSteps);
     j->bytes = estimateDenseBytes(N, m, denseFormat_);
     SQ_THROW_IF(j->bytes > budget_, "job needs ~%zu MiB, over the %zu MiB budget", j->bytes >> 20, budget_ >> 20);
+    /* the grouped sweep holds a job's fields in one wave's LDS region and
+     * needs its m slices resident at once: a job past either limit could
+     * never be stepped */
+    const int Np = (int)roundUp256(N);
+    SQ_THROW_IF(!denseGroupFits(Np, fp8RowStride(Np), m),
+                "dense job N=%d, m=%d does not fit the grouped sweep (LDS region per slice or resident waves)", N,
+                m);
     j->submitS = now();
     pending_.push_back(std::move(j));
     return pending_.back()->id;
@@ -182,41 +195,86 @@ int Server::submitBipartite(const float *b0, const float *b1, const float *W, in
     return pending_.back()->id;
 }
 
+bool Server::denseGroupFits(int maxNp, int maxLd8, int maxM) const {
+    const size_t region = denseSweepBatchRegionBytes(maxNp, maxLd8, sizeof(float), false, denseFormat_);
+    if (region + 256 > dev_->maxLdsPerBlock())
+        return false;
+    int capacity = denseSweepBatchBlocks<float>(region, 1, false, denseFormat_, dev_->numCUs());
+    if (blockFraction_ < 1.)
+        capacity = std::max(1, (int)(capacity * blockFraction_));
+    return capacity >= maxM;
+}
+
 void Server::admit() {
     while (!pending_.empty()) {
         Job &j = *pending_.front();
         if ((int)active_.size() >= maxActive_ || memInUse_ + j.bytes > budget_)
             break; /* arrival order: a job that does not fit holds the ones behind it */
-        if (j.kind == kDense) {
-            j.dense.reset(new DenseGraphAnnealer<float>());
-            auto &a = *j.dense;
-            a.assignDevice(*dev_);
-            a.seed(j.seed);
-            a.setQUBO(j.W.data(), j.N0, optMinimize);
-            a.setPreference("n_trotters", std::to_string(j.m));
-            a.setPreference("field_precision", formatName(denseFormat_));
-            a.prepare();
-            a.randomizeSpin();
-        } else {
-            j.bip.reset(new BipartiteGraphAnnealer<float>());
-            auto &a = *j.bip;
-            a.assignDevice(*pool_[nextStream_]);
-            nextStream_ = (nextStream_ + 1) % (int)pool_.size();
-            a.seed(j.seed);
-            a.setQUBO(j.b0.data(), j.b1.data(), j.W.data(), j.N0, j.N1, optMinimize);
-            a.setPreference("n_trotters", std::to_string(j.m));
-            a.setPreference("field_precision", formatName(bipartiteFormat_));
-            a.prepare();
-            a.randomizeSpin();
+        std::unique_ptr<Job> job = std::move(pending_.front());
+        pending_.erase(pending_.begin());
+        try {
+            if (j.kind == kDense) {
+                j.dense.reset(new DenseGraphAnnealer<float>());
+                auto &a = *j.dense;
+                a.assignDevice(*dev_);
+                a.seed(j.seed);
+                a.setQUBO(j.W.data(), j.N0, optMinimize);
+                a.setPreference("n_trotters", std::to_string(j.m));
+                a.setPreference("field_precision", formatName(denseFormat_));
+                if (denseSpinLimit_ > 0)
+                    a.setPreference("spin_limit", std::to_string(denseSpinLimit_));
+                a.prepare();
+                a.randomizeSpin();
+            } else {
+                j.bip.reset(new BipartiteGraphAnnealer<float>());
+                auto &a = *j.bip;
+                a.assignDevice(*pool_[nextStream_]);
+                nextStream_ = (nextStream_ + 1) % (int)pool_.size();
+                a.seed(j.seed);
+                a.setQUBO(j.b0.data(), j.b1.data(), j.W.data(), j.N0, j.N1, optMinimize);
+                a.setPreference("n_trotters", std::to_string(j.m));
+                a.setPreference("field_precision", formatName(bipartiteFormat_));
+                a.prepare();
+                a.randomizeSpin();
+            }
+        } catch (const std::exception &e) {
+            fail(j, e.what()); /* device state (if any) is freed with the job */
+            continue;
         }
         std::vector<float>().swap(j.W);
         std::vector<float>().swap(j.b0);
         std::vector<float>().swap(j.b1);
         j.admitS = now();
+        j.admitted = true;
         memInUse_ += j.bytes;
         memPeak_ = std::max(memPeak_, memInUse_);
-        active_.push_back(std::move(pending_.front()));
-        pending_.erase(pending_.begin());
+        active_.push_back(std::move(job));
+    }
+}
+
+void Server::stepDenseGroup(const std::vector<DenseGraphAnnealer<float> *> &dense, const std::vector<Job *> &jobs,
+                            const std::vector<int> &order, GroupedDenseStepper &stepper) {
+    GroupedDenseStepper::Options o;
+    o.fieldFormat = denseFormat_;
+    o.wavesPerBlock = 1;
+    o.graph = false;
+    o.wait = true;
+    o.blockFraction = blockFraction_;
+    try {
+        stepper.run(*dev_, dense, order, 1,
+                    [&](int, int i) {
+                        const Job *jb = jobs[i];
+                        return std::make_pair(jb->G[jb->step], jb->beta[jb->step]);
+                    },
+                    o);
+    } catch (const std::exception &e) {
+        /* jobs whose sweep aborted fail; if the step failed otherwise, none
+         * of the group's jobs has a valid state and they all fail; the other
+         * groups and the bipartite jobs go on */
+        const std::vector<int> &ab = stepper.lastAborted();
+        for (int i : (ab.empty() ? order : ab))
+            if (!jobs[i]->failed)
+                fail(*jobs[i], e.what());
     }
 }
 
@@ -226,8 +284,13 @@ int Server::tick() {
         return (int)pending_.size();
     /* bipartite steps first, on their own streams */
     for (auto &j : active_)
-        if (j->kind == kBipartite)
-            j->bip->annealOneStep((float)j->G[j->step], (float)j->beta[j->step]);
+        if (j->kind == kBipartite) {
+            try {
+                j->bip->annealOneStep((float)j->G[j->step], (float)j->beta[j->step]);
+            } catch (const std::exception &e) {
+                fail(*j, e.what());
+            }
+        }
     /* dense steps: one persistent sweep per size class, on the server's stream */
     std::vector<DenseGraphAnnealer<float> *> dense;
     std::vector<Job *> denseJobs;
@@ -245,32 +308,74 @@ int Server::tick() {
                        ? dense[x]->getProblemSize() > dense[y]->getProblemSize()
                        : dense[x]->numTrotters() > dense[y]->numTrotters();
         });
-        GroupedDenseStepper::Options o;
-        o.fieldFormat = denseFormat_;
-        o.wavesPerBlock = 1;
-        o.graph = false;
-        o.wait = true;
-        o.blockFraction = blockFraction_;
-        steppers_[c.first].run(*dev_, dense, order, 1,
-                               [&](int, int i) {
-                                   const Job *jb = denseJobs[i];
-                                   return std::make_pair(jb->G[jb->step], jb->beta[jb->step]);
-                               },
-                               o);
+        /* Every job fits the grouped sweep alone (checked at submission), but
+         * the largest N and the largest m of a class may not fit one launch
+         * together: such a class runs as consecutive groups, each as large
+         * as fits (jobs in the class's order). */
+        size_t g0 = 0;
+        while (g0 < order.size()) {
+            std::vector<int> group;
+            int maxNp = 0, maxLd8 = 0, maxM = 0;
+            for (size_t g = g0; g < order.size(); ++g) {
+                const DenseGraphAnnealer<float> &a = *dense[order[g]];
+                const int np = std::max(maxNp, a.paddedN()), l8 = std::max(maxLd8, a.fp8RowStrideBytes());
+                const int mm = std::max(maxM, a.numTrotters());
+                if (!group.empty() && !denseGroupFits(np, l8, mm))
+                    break;
+                group.push_back(order[g]);
+                maxNp = np;
+                maxLd8 = l8;
+                maxM = mm;
+            }
+            g0 += group.size();
+            stepDenseGroup(dense, denseJobs, group, steppers_[c.first]);
+        }
     }
     for (auto &d : pool_)
         d->synchronize();
-    /* advance; finish the jobs whose schedule is done */
-    std::vector<std::unique_ptr<Job>> still;
-    for (auto &j : active_) {
+    /* advance; finish the jobs whose schedule is done.  active_ is rebuilt
+     * from moved-out entries only after every job has been handled, and a job
+     * whose solution throws becomes a failed result, so a failure leaves no
+     * job half-moved. */
+    std::vector<std::unique_ptr<Job>> all;
+    all.swap(active_);
+    for (auto &j : all) {
+        if (j->failed)
+            continue; /* its result is recorded */
         ++j->step;
-        if (j->step >= (int)j->G.size())
-            finish(*j);
-        else
-            still.push_back(std::move(j));
+        if (j->step >= (int)j->G.size()) {
+            try {
+                finish(*j);
+            } catch (const std::exception &e) {
+                fail(*j, e.what());
+            }
+        } else {
+            active_.push_back(std::move(j));
+        }
     }
-    active_.swap(still);
     return (int)(active_.size() + pending_.size());
+}
+
+void Server::fail(Job &j, const char *what) {
+    Result r;
+    r.kind = j.kind;
+    r.N0 = j.N0;
+    r.N1 = j.N1;
+    r.m = j.m;
+    r.attempts = 0;
+    r.submitS = j.submitS;
+    r.admitS = j.admitS;
+    r.finishS = now();
+    r.error = (what && *what) ? what : "job failed";
+    j.dense.reset();
+    j.bip.reset();
+    if (j.admitted) {
+        memInUse_ -= j.bytes;
+        j.admitted = false;
+    }
+    j.failed = true;
+    done_[j.id] = std::move(r);
+    newlyDone_.push_back(j.id);
 }
 
 void Server::finish(Job &j) {
@@ -300,6 +405,7 @@ void Server::finish(Job &j) {
     r.admitS = j.admitS;
     r.finishS = now();
     memInUse_ -= j.bytes;
+    j.admitted = false;
     done_[j.id] = std::move(r);
     newlyDone_.push_back(j.id);
 }

@@ -285,12 +285,16 @@ public:
     void run(Device &dev, const std::vector<DenseGraphAnnealer<float> *> &inst, const std::vector<int> &order,
              int nSteps, const Schedule &sched, const Options &o);
     double lastMs() const { return lastMs_; }
+    /* instances whose sweep aborted (spin limit) in the last run: run()
+     * checks every instance and then throws once if this is not empty */
+    const std::vector<int> &lastAborted() const { return lastAborted_; }
 
 private:
     DevBuf<unsigned char> d_plan_;
     PinnedBuf<unsigned char> h_plan_;
     DevBuf<int> d_queue_;
     double lastMs_ = 0.;
+    std::vector<int> lastAborted_;
 };
 
 class DenseBatch {
@@ -556,11 +560,12 @@ public:
         std::vector<int8_t> x0, x1; /* dense: x0 is m x N */
         double submitS, admitS, finishS; /* seconds on the server's clock */
         long long attempts;
+        std::string error; /* non-empty: the job failed (admission, a step or its solution threw) */
     };
     explicit Server(Device &dev);
     ~Server();
     /* memory_budget_mb, max_active, dense_field_precision, bipartite_field_precision,
-     * bipartite_streams, block_fraction */
+     * bipartite_streams, block_fraction, dense_spin_limit */
     void setPreference(const std::string &name, const std::string &value);
     int submitDense(const float *W, int N, int m, unsigned long long seed, const double *G, const double *beta,
                     int nSteps);
@@ -584,6 +589,10 @@ private:
     struct Job;
     void admit();
     void finish(Job &j);
+    void fail(Job &j, const char *what);
+    void stepDenseGroup(const std::vector<DenseGraphAnnealer<float> *> &dense, const std::vector<Job *> &jobs,
+                        const std::vector<int> &order, GroupedDenseStepper &stepper);
+    bool denseGroupFits(int maxNp, int maxLd8, int maxM) const;
 
     Device *dev_;
     std::vector<std::unique_ptr<Device>> pool_;
@@ -591,6 +600,7 @@ private:
     int maxActive_ = 256;
     int denseFormat_ = fieldBF16x2, bipartiteFormat_ = fieldBF16x2;
     double blockFraction_ = 1.0;
+    int denseSpinLimit_ = 0; /* 0: the annealer's default */
     int nextId_ = 0, nextStream_ = 0;
     size_t memInUse_ = 0, memPeak_ = 0;
     std::vector<std::unique_ptr<Job>> pending_, active_;

@@ -25,7 +25,7 @@ _PP = ctypes.POINTER(ctypes.c_void_p)
 
 _SIGS = {
     "sqaod_rocm_last_error": (ctypes.c_char_p, []),
-    "sqaod_rocm_version": (None, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "sqaodc_rocm_version": (None, [ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "sqaod_rocm_device_count": (_i32, [ctypes.POINTER(_i32)]),
     "sqaod_rocm_device_new": (_i32, [_i32, _PP]),
     "sqaod_rocm_device_delete": (_i32, [_vp]),
@@ -153,6 +153,7 @@ _SIGS = {
                                               ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "sqaod_rocm_server_result": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_longlong)]),
     "sqaod_rocm_server_release": (_i32, [_vp, _i32]),
+    "sqaod_rocm_server_result_error": (_i32, [_vp, _i32, ctypes.c_char_p, _sz, ctypes.POINTER(_i32)]),
     "sqaod_rocm_server_now": (_i32, [_vp, ctypes.POINTER(_f64)]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -206,8 +207,10 @@ def _get_prefs(getter, cobj):
 
 
 def version():
+    """(sqaodc API version, HIP version), as sqaodc_cuda_version reports
+    (sqaodc version, CUDA version)."""
     a, b = _i32(), _i32()
-    _lib.sqaod_rocm_version(ctypes.byref(a), ctypes.byref(b))
+    _lib.sqaodc_rocm_version(ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
 
 
@@ -940,6 +943,14 @@ class _Server:
         else:
             out["x0"], out["x1"] = x0, x1
         return out
+
+    @staticmethod
+    def error(cobj, i):
+        """The failed job's message, or None for a job that completed."""
+        buf = ctypes.create_string_buffer(1024)
+        failed = _i32()
+        _check(_lib.sqaod_rocm_server_result_error(cobj, i, buf, len(buf), ctypes.byref(failed)))
+        return buf.value.decode() if failed.value else None
 
     @staticmethod
     def release(cobj, i):

@@ -45,7 +45,14 @@ class Server:
         return cext.take_done(self._cobj)
 
     def result(self, job):
+        """E, x (or x0, x1), times and attempts of a finished job; raises
+        RuntimeError with the job's message if it failed."""
         return cext.result(self._cobj, job)
+
+    def error(self, job):
+        """None for a job that completed, else why it failed (admission, a
+        step or its solution threw); the server went on with the others."""
+        return cext.error(self._cobj, job)
 
     def release(self, job):
         cext.release(self._cobj, job)

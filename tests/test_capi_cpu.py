@@ -20,7 +20,7 @@ def cext():
 
 def declared_symbols():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(sqaod_rocm_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b((?:sqaod_rocm|sqaodc_rocm)_\w+)\s*\(", txt)))
 
 
 def test_header_symbols_exported(cext):
@@ -31,8 +31,18 @@ def test_header_symbols_exported(cext):
     assert set(syms) == set(cext.EXPORTED)
 
 
-def test_version(cext):
-    assert cext.version() == (1, 0)
+def test_version_probe_like_the_cuda_library(cext):
+    """sqaodc_rocm_version(int *version, int *hip_version), the shape of
+    sqaodc_cuda_version (sqaodc/cuda/api_cuda.cpp:5-9), read the way
+    sqaodpy/sqaod/common/envcheck.py:27-37 reads the CUDA probe."""
+    import ctypes
+    so = ctypes.CDLL(cext.LIB_PATH)
+    assert hasattr(so, "sqaodc_rocm_version")
+    ver, hip_ver = ctypes.c_int(0), ctypes.c_int(0)
+    so.sqaodc_rocm_version(ctypes.byref(ver), ctypes.byref(hip_ver))
+    assert ver.value == 10003
+    assert hip_ver.value >= 70000000  # ROCm 7 HIP: major * 10^7 + minor * 10^5 + patch
+    assert cext.version() == (ver.value, hip_ver.value)
 
 
 def test_state_machine_errors_without_gpu(cext):
@@ -142,3 +152,35 @@ def test_bf_searcher_host_logic_without_gpu(cext):
             B.set_preferences(h, {"tile_size": 0}, np.float64)
     finally:
         B.delete(h, np.float64)
+
+
+def test_cpp_interface_header_compiles(tmp_path):
+    """include/sqaod_rocm.hpp: every solver of the Solver.h-shaped interface,
+    for float and double, instantiated in full (all virtual methods), and the
+    reference's calling idioms (setPreference(name, value), selectAlgorithm,
+    deleteInstance) compile without warnings."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    kinds = ["DenseGraphAnnealerImpl", "BipartiteGraphAnnealerImpl", "DenseGraphBFSearcherImpl",
+             "BipartiteGraphBFSearcherImpl", "DenseGraphFormulasImpl", "BipartiteGraphFormulasImpl"]
+    body = "".join(f"template class sqaod::rocm::detail::{k}<{t}>;\n" for k in kinds for t in ("float", "double"))
+    body += """
+void use() {
+    namespace sr = sqaod::rocm;
+    sr::Device *d = sr::newDevice();
+    sr::DenseGraphAnnealer<double> *a = sr::newDenseGraphAnnealer<double>();
+    a->setPreference(sr::pnNumTrotters, 8);
+    a->setPreference(sr::pnAlgorithm, sr::algoColoring);
+    a->selectAlgorithm(sr::algoSANaive);
+    sr::BipartiteGraphBFSearcher<float> *b = sr::newBipartiteGraphBFSearcher<float>();
+    sr::PackedBitSet x0 = 0, x1 = 0;
+    (void)b->searchRange(&x0, &x1);
+    sr::deleteInstance(a);
+    sr::deleteInstance(b);
+    sr::deleteInstance(d);
+}
+"""
+    src.write_text('#include "sqaod_rocm.hpp"\n' + body)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

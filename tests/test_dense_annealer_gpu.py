@@ -232,3 +232,28 @@ def test_largest_lds_resident_n_bit_exact(oracle, rocm, form):
         an.anneal_one_step(G, BETA)
         qa, qo = q_of(an), ora.get_q()
         assert np.array_equal(qa, qo), f"{np.sum(qa != qo)} spins differ"
+
+
+def test_spin_limit_gives_up_cleanly(rocm):
+    """With a spin limit of one blocked iteration (forced from the host) the
+    sweep's bounded wait gives up instead of waiting: the step raises
+    RuntimeError, every kernel has exited, and the annealer works again once
+    its spins are set anew."""
+    W = dense_int_W(128, SEED_BASE + 41)
+    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=2, spin_limit=1)
+    an.seed(3)
+    an.prepare()
+    an.randomize_spin()
+    with pytest.raises(RuntimeError, match="spin limit"):
+        for G in schedule()[:40]:
+            an.anneal_one_step(G, BETA)
+        an.get_q()  # a flag written by the last step is reported here at the latest
+    with pytest.raises(RuntimeError):
+        an.set_preferences(spin_limit=0)
+    an.set_preferences(spin_limit=1 << 20)
+    an.randomize_spin()
+    for G in schedule()[:10]:
+        an.anneal_one_step(G, BETA)
+    an.make_solution()
+    x = np.stack(an.get_x()).astype(np.float64)
+    assert np.array_equal(an.get_E(), np.einsum("yi,yi->y", x @ W.astype(np.float64), x).astype(np.float32))

@@ -129,3 +129,51 @@ def test_rejects_job_over_budget_and_bad_input(rocm):
         srv.submit_dense(np.arange(16, dtype=np.float32).reshape(4, 4), 4, 1, sched(1.0, 3), BETA)
     with pytest.raises(RuntimeError):
         srv.set_preferences(dense_field_precision="fp16")
+
+
+def test_rejects_dense_job_the_grouped_sweep_cannot_hold(rocm):
+    """A dense job whose slices cannot all be resident (m above the waves the
+    sweep may use) or whose fields do not fit a wave's LDS region (N above
+    the LDS-resident limit) is rejected at submission instead of failing
+    every tick."""
+    srv = rocm.server(block_fraction=0.001)  # leaves a single resident wave
+    with pytest.raises(RuntimeError, match="grouped sweep"):
+        srv.submit_dense(dense_int_W(64, 2), 16, 1, sched(1.0, 3), BETA)
+    srv = rocm.server()
+    N = 11777  # padded to 12032 sites, over the LDS-resident limit of the sweep
+    with pytest.raises(RuntimeError, match="grouped sweep"):
+        srv.submit_dense(np.zeros((N, N), np.float32), 16, 1, sched(1.0, 3), BETA)
+    assert srv.status()["pending"] == 0
+
+
+def test_failed_job_leaves_the_others_running(rocm):
+    """A dense job whose sweep gives up (a spin limit of one blocked
+    iteration, forced from the host; with m = 2 both neighbours are one slice
+    and its slices wait on each other often) fails with a recorded error,
+    its memory is returned, and the jobs stepped beside it -- a dense job in
+    the same launch and a bipartite job -- still equal the same jobs run
+    alone."""
+    js = jobs()
+    good_dense = js[0]  # N = 64, m = 16, 30 steps, integer W
+    good_bip = js[6]    # bipartite 100 x 80, m = 6
+    srv = rocm.server(dense_field_precision="bf16x2", bipartite_field_precision="bf16x2")
+    a = submit(srv, good_dense)
+    b = submit(srv, good_bip)
+    srv.tick()
+    srv.set_preferences(dense_spin_limit=1)
+    bad = srv.submit_dense(dense_int_W(128, 5), 2, 77, sched(5.0, 30), BETA)
+    while srv.tick() > 0:
+        pass
+    done = set(srv.take_done())
+    assert done == {a, b, bad}
+    err = srv.error(bad)
+    assert err is not None and "spin limit" in err
+    with pytest.raises(RuntimeError, match="failed"):
+        srv.result(bad)
+    assert srv.error(a) is None and srv.error(b) is None
+    assert srv.status()["bytes_in_use"] == 0
+    for jid, job in ((a, good_dense), (b, good_bip)):
+        kind, data, m, seed, G = job
+        E, xs = alone(rocm, kind, data, m, seed, G)
+        r = srv.result(jid)
+        assert np.array_equal(r["E"], E), kind
