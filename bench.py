@@ -33,20 +33,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dense_sweep_bf16x2_n4096_m256.md")
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+FP8_MFMA_PEAK_TFLOPS = 5000.0
+PROFILES = os.path.join(ROOT, "profiles")
+BETA = 50.0
 
 
-def measured_traffic(N, m, form):
-    """HBM bytes per sweep launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB copy) of this exact
-    workload; None for any other workload or form."""
-    if (N, m, form) != (4096, 256, "bf16x2") or not os.path.exists(TRAFFIC_PROFILE):
+def traffic_profile_path(N, m, form, steps, warmup):
+    """The committed rocprofv3 PMC summary of exactly this bench window."""
+    return os.path.join(PROFILES, f"dense_sweep_{form}_n{N}_m{m}_steps0-{steps - 1}_warmup{warmup}.md")
+
+
+def measured_traffic(N, m, form, steps, warmup):
+    """HBM bytes per sweep launch, averaged over the timed launches, from the
+    committed rocprofv3 PMC summary of this exact workload and window
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated against a device copy of known
+    size); None when no such profile exists."""
+    path = traffic_profile_path(N, m, form, steps, warmup)
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_PROFILE):
+    for line in open(path):
         if line.startswith("sweep_hbm_bytes_per_launch:"):
             return int(line.split(":", 1)[1])
     return None
-BETA = 50.0
+
+
+def sweep_bytes_per_launch(N, m, form, flips_per_launch):
+    """Bytes the local-field sweep must move in one launch (one annealOneStep,
+    N m attempts): per attempt the site and uniform it reads (4 + 4 B), the
+    decision word it publishes (4 B) and the two neighbour slices' reads of
+    it (2 x 4 B); per slice its fields in and out and h (3 x 4 Np B) and its
+    spin bits in and out plus the two neighbour mirrors (4 x Np / 8 B); per
+    accepted flip one J row (4 Np B as fp32 / bf16 hi+lo sums, Np + 4 B as
+    e4m3 bytes and the row scale)."""
+    Np = (N + 255) // 256 * 256
+    row = (Np + 1023) // 1024 * 1024 + 4 if form == "fp8" else 4 * Np
+    return N * m * 20 + m * (12 * Np + Np // 2) + flips_per_launch * row
 
 
 def schedule_G(k):
@@ -116,7 +138,19 @@ def run_form(rocm, W, m, form, steps, warmup, seed):
         "E_fp64_rounding_max_abs": Eerr,
         "E_form_max_abs_err": Eform,
         "E_form_max_rel_err": Eform / max(abs(float(np.min(E))), 1e-30),
+        "flips_per_launch": flips / max(sweep_n, 1),
     }
+    if form != "plain":
+        # the field GEMM's own time: HIP events around every phase, in a pass
+        # after the timed region (the events would add gaps to the timed steps)
+        an.set_preferences(profile=2)
+        an.get_phase_times()
+        for k in range(steps, steps + 10):
+            an.anneal_one_step(schedule_G(k), BETA)
+        dev.synchronize()
+        g_ms, g_n = an.get_phase_times()["refresh"]
+        if g_n:
+            res["field_gemm_avg_ms"] = g_ms / g_n
     del an
     return res
 
@@ -389,8 +423,13 @@ def main():
     head = "bf16x2" if "bf16x2" in results else forms[0]
     r = results[head]
     B_att = bytes_per_attempt(N, head)
-    per_launch = B_att * N * m
-    achieved = per_launch / (r["sweep_avg_ms"] * 1e-3) / 1e9
+    ref_per_launch = B_att * N * m
+    sweep_s = r["sweep_avg_ms"] * 1e-3
+    algo_per_launch = sweep_bytes_per_launch(N, m, head, r["flips_per_launch"]) if head != "plain" else ref_per_launch
+    achieved = algo_per_launch / sweep_s / 1e9
+    traffic = measured_traffic(N, m, head, args.steps, args.warmup)
+    window = (f"schedule steps 0-{args.steps - 1} of 205 (after {args.warmup} warm-up steps and re-randomized "
+              f"spins)")
     out = {
         "metric": "spin-flips/s dense SQA N=4096 m=256 bf16-split vs fp32; serve p50/p99 ms",
         "value": r["attempts_per_s"],
@@ -406,8 +445,8 @@ def main():
         "data": "synthetic: W symmetric i.i.d. N(0,1) from a seeded generator",
         "config": {
             "workload": f"dense SQA annealOneStep, N={N}, m={m}, G=5.0*0.97^k, beta=50, "
-                        f"local fields, J as bf16 hi+lo, Philox streams",
-            "N": N, "m": m, "form": head,
+                        f"local fields, J as bf16 hi+lo, Philox streams; timed: {window}",
+            "N": N, "m": m, "form": head, "window": window,
         },
         "roofline": {
             "bound": "hbm",
@@ -415,14 +454,39 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(N, m, head),
+            "traffic": traffic,
+            "traffic_frac": (traffic / sweep_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
             "kernel": "denseSweepKernel",
-            "note": f"achieved = algorithmic bytes per launch N*m*(4N+N+8) = {per_launch} (the reference "
-                    f"algorithm's J-row + q-row + stream per attempt) / measured sweep time; traffic = "
-                    f"measured HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, profiles/)",
+            "algorithmic_bytes_per_launch": algo_per_launch,
+            "note": "achieved = the local-field sweep's algorithmic bytes per launch (stream, decisions, fields, "
+                    "spins, one J row per accepted flip at the measured flip count) / the sweep's average "
+                    "duration (HIP events on the solver's stream); traffic = measured HBM bytes per launch of "
+                    "this window (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/), traffic_frac = traffic / "
+                    "duration / peak",
+        },
+        "reference_work_equivalent": {
+            "bytes_per_launch": ref_per_launch,
+            "GBps": ref_per_launch / sweep_s / 1e9,
+            "note": "N*m*(4N+N+8): the bytes the reference algorithm (a J-row . q-row dot per attempt) "
+                    "would move for the same attempts; not bytes this sweep moves",
         },
         "forms": results,
     }
+    gemm_ms = r.get("field_gemm_avg_ms")
+    if gemm_ms:
+        planes = {"bf16x2": 2, "fp32": 3, "fp8": 1}[head]
+        flops = 2.0 * m * N * N
+        peak = FP8_MFMA_PEAK_TFLOPS if head == "fp8" else BF16_MFMA_PEAK_TFLOPS
+        out["field_gemm"] = {
+            "kernel": "fieldGemm (F = Q J on MFMA)", "avg_ms": gemm_ms,
+            "algorithmic_TFLOPs": flops / (gemm_ms * 1e-3) / 1e12,
+            "mfma_issued_TFLOPs": planes * flops / (gemm_ms * 1e-3) / 1e12,
+            "peak_TFLOPs": peak,
+            "mfma_frac": planes * flops / (gemm_ms * 1e-3) / 1e12 / peak,
+            "note": f"2 m N^2 = {flops:.4g} flop per refresh, issued {planes}x on "
+                    f"{'fp8' if head == 'fp8' else 'bf16'} MFMA ({planes} J planes); time from HIP events around "
+                    f"the refresh in 10 steps after the timed region",
+        }
     if "plain" in results and head in results:
         out["speedup_vs_plain_fp32"] = r["attempts_per_s"] / results["plain"]["attempts_per_s"]
     if "fp32" in results and head in results:
@@ -446,6 +510,12 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(W, m, args.cpu_seconds, seed=1000)
     print(json.dumps(out), flush=True)
+    # release every device object while the runtime (and any profiler attached
+    # to it) is fully up, rather than during interpreter teardown
+    import gc
+    gc.collect()
+    rocm.device.unload()
+    gc.collect()
 
 
 if __name__ == "__main__":

@@ -290,7 +290,7 @@ double sqo_dot_f64(const double *v0, const double *v1, int n) {
 void sqo_dg_energy_q_f64acc(int N, int m, const double *h, const double *J, double c,
                             const int8_t *q, double *E) {
     /* E_y = -c - h.q_y - q_y^T J q_y   (SharedFormulas.cpp:70-83) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((size_t)m * N * N > ((size_t)1 << 18))
     for (int y = 0; y < m; ++y) {
         const int8_t *qy = q + (size_t)y * N;
         double hq = 0., qJq = 0.;
@@ -308,7 +308,7 @@ void sqo_dg_energy_q_f64acc(int N, int m, const double *h, const double *J, doub
 
 void sqo_dg_energy_x_f64acc(int N, int nx, const double *W, const int8_t *x, double *E) {
     /* E = x^T W x  (SharedFormulas.cpp:23-32) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if ((size_t)nx * N * N > ((size_t)1 << 18))
     for (int k = 0; k < nx; ++k) {
         const int8_t *xk = x + (size_t)k * N;
         double e = 0.;

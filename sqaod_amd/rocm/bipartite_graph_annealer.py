@@ -1,7 +1,7 @@
 """rocm bipartite graph annealer (sqaodpy/sqaod/cuda/bipartite_graph_annealer.py)."""
 import numpy as np
 
-from ..common.bipartite_graph_annealer_base import BipartiteGraphAnnealerBase
+from ..common.front import BipartiteGraphAnnealerBase
 from ..common.preference import minimize
 from ._cext import rocm_bg_annealer as cext
 from . import device
@@ -15,7 +15,7 @@ class BipartiteGraphAnnealer(BipartiteGraphAnnealerBase):
         dev = device.get_active_device()
         cext.assign_device(self._cobj, dev._cobj, dtype)
         self._device = dev
-        BipartiteGraphAnnealerBase.__init__(self, cext, dtype, b0, b1, W, optimize, prefdict)
+        BipartiteGraphAnnealerBase.__init__(self, cext, dtype, (b0, b1, W), optimize, prefdict)
 
 
 def bipartite_graph_annealer(b0=None, b1=None, W=None, optimize=minimize, dtype=np.float64, **prefs):

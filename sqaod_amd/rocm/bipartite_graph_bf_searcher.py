@@ -2,7 +2,7 @@
 (sqaodpy/sqaod/cuda/bipartite_graph_bf_searcher.py)."""
 import numpy as np
 
-from ..common.bipartite_graph_bf_searcher_base import BipartiteGraphBFSearcherBase
+from ..common.front import BipartiteGraphBFSearcherBase
 from ..common.preference import minimize
 from ._cext import rocm_bg_bf_searcher as cext
 from . import device
@@ -16,7 +16,7 @@ class BipartiteGraphBFSearcher(BipartiteGraphBFSearcherBase):
         dev = device.get_active_device()
         cext.assign_device(self._cobj, dev._cobj, dtype)
         self._device = dev
-        BipartiteGraphBFSearcherBase.__init__(self, cext, dtype, b0, b1, W, optimize, prefdict)
+        BipartiteGraphBFSearcherBase.__init__(self, cext, dtype, (b0, b1, W), optimize, prefdict)
 
 
 def bipartite_graph_bf_searcher(b0=None, b1=None, W=None, optimize=minimize, dtype=np.float64, **prefs):

@@ -1,7 +1,7 @@
 """rocm dense graph annealer (sqaodpy/sqaod/cuda/dense_graph_annealer.py:1-34)."""
 import numpy as np
 
-from ..common.dense_graph_annealer_base import DenseGraphAnnealerBase
+from ..common.front import DenseGraphAnnealerBase
 from ..common.preference import minimize
 from ._cext import rocm_dg_annealer as cext
 from . import device
@@ -15,7 +15,7 @@ class DenseGraphAnnealer(DenseGraphAnnealerBase):
         dev = device.get_active_device()
         cext.assign_device(self._cobj, dev._cobj, dtype)
         self._device = dev
-        DenseGraphAnnealerBase.__init__(self, cext, dtype, W, optimize, prefdict)
+        DenseGraphAnnealerBase.__init__(self, cext, dtype, (W,), optimize, prefdict)
 
     def get_counters(self):
         """Accepted flips, committed batches and blocked waits since prepare()."""

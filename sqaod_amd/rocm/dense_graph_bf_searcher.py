@@ -2,7 +2,7 @@
 (sqaodpy/sqaod/cuda/dense_graph_bf_searcher.py)."""
 import numpy as np
 
-from ..common.dense_graph_bf_searcher_base import DenseGraphBFSearcherBase
+from ..common.front import DenseGraphBFSearcherBase
 from ..common.preference import minimize
 from ._cext import rocm_dg_bf_searcher as cext
 from . import device
@@ -16,7 +16,7 @@ class DenseGraphBFSearcher(DenseGraphBFSearcherBase):
         dev = device.get_active_device()
         cext.assign_device(self._cobj, dev._cobj, dtype)
         self._device = dev
-        DenseGraphBFSearcherBase.__init__(self, cext, dtype, W, optimize, prefdict)
+        DenseGraphBFSearcherBase.__init__(self, cext, dtype, (W,), optimize, prefdict)
 
 
 def dense_graph_bf_searcher(W=None, optimize=minimize, dtype=np.float64, **prefs):
