@@ -76,7 +76,8 @@ int sqaod_rocm_dg_annealer_set_hamiltonian(sqaod_rocm_dg_annealer *a, const void
 /* annealer_get_problem_size (:163-196) */
 int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
 /* annealer_set_preferences (:279-299), one preference per call, value as text:
- * "algorithm" ("coloring" | "sa_naive" | "naive" | "default" | ...),
+ * "algorithm" ("coloring" | "naive" | "sa_naive" | "default" | "sa_default";
+ * others fall back to the default of their kind, Solver.cpp:152-168),
  * "n_trotters" (int), and the rocm-device knobs "rng" ("philox" | "mt19937"),
  * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2" | "fp8"),
  * "slices_per_block" (int, 0 = auto), "sweep_launch" ("auto" | "cooperative"),
@@ -121,6 +122,17 @@ int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, 
  * max |E_dtype - E_fp64| of the last energy evaluation. */
 int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]);
 int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err);
+/* System-energy trace, for callers that record getSystemE after every step
+ * (sqaodpy/sqaod/wildqat/opt.py:33-35, 57-62 calls get_system_E per step);
+ * no reference counterpart.  With capacity > 0, every anneal_one_step(G,
+ * beta) appends the value get_system_E(G, beta) would return for the state
+ * after the step, computed on the device without a host synchronisation
+ * (the first `capacity` steps are kept); 0 turns the trace off.  Setting it
+ * clears the trace.  get_system_E_trace copies up to cap kept values (as
+ * double) into out, *n = how many, *steps = steps traced since it was set. */
+int sqaod_rocm_dg_annealer_set_system_E_trace(sqaod_rocm_dg_annealer *a, int capacity);
+int sqaod_rocm_dg_annealer_get_system_E_trace(sqaod_rocm_dg_annealer *a, double *out, int cap, int *n,
+                                              long long *steps);
 /* Diagnostic without a reference counterpart: max over slices of
  * |E_form(q_y) - E(q_y)|, both in FP64, for the current spins, where E_form
  * uses the J the field precision form anneals (bf16 hi + lo or the fp8 J')

@@ -419,6 +419,15 @@ int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, 
     return guarded([&] { withDg(a, [&](auto &s) { *err = s.lastEnergyRoundingError(); }); });
 }
 
+int sqaod_rocm_dg_annealer_set_system_E_trace(sqaod_rocm_dg_annealer *a, int capacity) {
+    return guarded([&] { withDg(a, [&](auto &s) { s.setSystemETrace(capacity); }); });
+}
+
+int sqaod_rocm_dg_annealer_get_system_E_trace(sqaod_rocm_dg_annealer *a, double *out, int cap, int *n,
+                                              long long *steps) {
+    return guarded([&] { withDg(a, [&](auto &s) { *n = s.getSystemETrace(out, cap, steps); }); });
+}
+
 int sqaod_rocm_dg_annealer_get_form_energy_error(sqaod_rocm_dg_annealer *a, double *err) {
     return guarded([&] { withDg(a, [&](auto &s) { *err = s.formEnergyError(); }); });
 }

@@ -244,19 +244,31 @@ void DenseGraphAnnealer<real>::seed(unsigned long long s) {
     sweepIdx_ = 0;
     if (d_mt_.ptr) {
         ensureDevice();
-        MTState st;
-        mtSeedHost(&st, s);
-        HIP_CHECK(hipMemcpyAsync(d_mt_.ptr, &st, sizeof(MTState), hipMemcpyHostToDevice, dev_->stream()));
-        dev_->synchronize();
+        seedWorkerGenerators();
     }
     setState(solRandSeedGiven);
 }
 
 template <class real>
+void DenseGraphAnnealer<real>::seedWorkerGenerators() {
+    /* worker w of the CPU annealer is seeded s + 17 w (:29-33) */
+    std::vector<MTState> st((size_t)nWorkers_);
+    for (int w = 0; w < nWorkers_; ++w)
+        mtSeedHost(&st[w], seed_ + 17ull * (unsigned long long)w);
+    d_mt_.allocate((size_t)nWorkers_);
+    HIP_CHECK(hipMemcpyAsync(d_mt_.ptr, st.data(), st.size() * sizeof(MTState), hipMemcpyHostToDevice,
+                             dev_->stream()));
+    dev_->synchronize();
+    mtWorkersSeeded_ = nWorkers_;
+}
+
+template <class real>
 Algorithm DenseGraphAnnealer<real>::selectAlgorithm(Algorithm algo) {
-    /* CUDADenseGraphAnnealer.cu:115-126: coloring and sa_naive are offered;
-     * other requests map to the default of their kind (Solver.cpp:152-168). */
+    /* CPUDenseGraphAnnealer.cpp:35-50: naive, coloring and sa_naive are
+     * offered; other requests map to the default of their kind
+     * (Solver.cpp:152-168). */
     switch (algo) {
+    case algoNaive:
     case algoColoring:
     case algoSANaive:
         algo_ = algo;
@@ -354,6 +366,15 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
             profile_ = 1;
         else
             profile_ = atoi(value.c_str()) < 0 ? 0 : atoi(value.c_str());
+    } else if (name == "n_workers") {
+        /* with rng=mt19937: the number of CPU workers (OpenMP threads) of the
+         * sqaod.cpu run whose coloring trajectory is replayed */
+        const int v = atoi(value.c_str());
+        SQ_THROW_IF(v < 1 || v > 1024, "n_workers must be in 1..1024");
+        if (v != nWorkers_) {
+            nWorkers_ = v;
+            clearState(solPrepared);
+        }
     } else if (name == "spin_limit") {
         const int v = atoi(value.c_str());
         SQ_THROW_IF(v <= 0, "spin_limit must be positive");
@@ -376,6 +397,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("precision", sizeof(real) == 4 ? "float" : "double");
     p.emplace_back("device", "rocm");
     p.emplace_back("rng", rngMT_ ? "mt19937" : "philox");
+    p.emplace_back("n_workers", std::to_string(nWorkers_));
     p.emplace_back("sweep", plainSweep_ ? "plain" : "local");
     p.emplace_back("field_precision", fieldFormat_ == fieldBF16x2 ? "bf16x2"
                                       : fieldFormat_ == fieldFP8  ? "fp8"
@@ -535,12 +557,17 @@ void DenseGraphAnnealer<real>::prepare() {
     HIP_CHECK(hipHostGetDevicePointer((void **)&d_abortHost_, h_abort_.ptr, 0));
     abortPending_ = false;
     if (rngMT_) {
-        d_mt_.allocate(1);
-        MTState st;
-        mtSeedHost(&st, seed_);
-        HIP_CHECK(hipMemcpyAsync(d_mt_.ptr, &st, sizeof(MTState), hipMemcpyHostToDevice, dev_->stream()));
-        const int perAttempt = sizeof(real) == 4 ? 2 : 3;
-        d_words_.allocate(mN * perAttempt);
+        /* the generators continue across prepare() calls, as the CPU
+         * annealer's do; they are (re)seeded by seed() and on first use */
+        if (!d_mt_.ptr || mtWorkersSeeded_ != nWorkers_)
+            seedWorkerGenerators();
+        /* the naive step draws one word more per attempt (its slice); the
+         * worker streams of the OpenMP order each hold a step's draws */
+        const int perAttempt = sizeof(real) == 4 ? 3 : 4;
+        size_t words = mN * perAttempt;
+        if (nWorkers_ > 1)
+            words = std::max<size_t>(words, (size_t)nWorkers_ * mtWorkerStride(N_, m_, nWorkers_, perAttempt - 1));
+        d_words_.allocate(words);
     }
     E_.assign((size_t)m_, real(0.));
     q_.assign(mN, 0);
@@ -620,8 +647,8 @@ void DenseGraphAnnealer<real>::checkAbort(bool synchronous) {
 }
 
 template <class real>
-void DenseGraphAnnealer<real>::refreshFields() {
-    if (plainSweep_) {
+void DenseGraphAnnealer<real>::refreshFields(bool always) {
+    if (plainSweep_ && !always) {
         fieldsDirty_ = false;
         return;
     }
@@ -658,7 +685,9 @@ void DenseGraphAnnealer<real>::refreshFields() {
 template <class real>
 void DenseGraphAnnealer<real>::generateAttempts() {
     const int order = (algo_ == algoSANaive) ? orderSliceMajor : orderColoring;
-    if (rngMT_) {
+    if (rngMT_ && nWorkers_ > 1 && algo_ == algoColoring) {
+        workerStreamsToAttempts<real>(d_mt_.ptr, d_words_.ptr, N_, m_, nWorkers_, d_X_.ptr, d_U_.ptr, dev_->stream());
+    } else if (rngMT_) {
         const int perAttempt = sizeof(real) == 4 ? 2 : 3;
         mtGenerate(d_mt_.ptr, d_words_.ptr, (size_t)m_ * N_ * perAttempt, dev_->stream());
         streamToAttempts<real>(d_words_.ptr, N_, m_, order, d_X_.ptr, d_U_.ptr, dev_->stream());
@@ -717,6 +746,48 @@ SweepArgs<real> DenseGraphAnnealer<real>::makeSweepArgs(real G, real beta) {
 }
 
 template <class real>
+void DenseGraphAnnealer<real>::annealOneStepNaive(real G, real beta) {
+    /* CPUDenseGraphAnnealer.cpp:268-280 on one workgroup (dense_naive.hip).
+     * The step keeps local fields like the coloring sweep; the plain sweep
+     * does not maintain them, so after plain steps they are recomputed. */
+    if (plainSweep_)
+        fieldsDirty_ = true;
+    if (!fieldsDirty_ && refreshInterval_ > 0 && stepsSinceRefresh_ >= refreshInterval_)
+        fieldsDirty_ = true;
+    if (fieldsDirty_)
+        refreshFields(true);
+    NaiveArgs<real> a{};
+    a.N = N_;
+    a.Np = Np_;
+    a.m = m_;
+    a.Nw = Nw_;
+    /* host scalars in the reference's expression order, in real (:272-273) */
+    a.twoDivM = real(2.) / real(m_);
+    a.coef = std::log(std::tanh(G * beta / real(m_))) / beta;
+    a.beta = beta;
+    a.J = (fieldFormat_ != fieldF32) ? reinterpret_cast<const real *>(d_Jsplit_.ptr) : d_J_.ptr;
+    a.h = d_h_.ptr;
+    a.F = d_F_.ptr;
+    a.Qbits = d_Q_.ptr;
+    a.counters = d_counters_.ptr;
+    if (rngMT_) {
+        const int words = sizeof(real) == 4 ? 3 : 4;
+        mtGenerate(d_mt_.ptr, d_words_.ptr, (size_t)m_ * N_ * words, dev_->stream());
+        a.words = d_words_.ptr;
+    } else {
+        const unsigned long long s = sweepIdx_++;
+        a.words = nullptr;
+        a.k0 = (uint32_t)seed_;
+        a.k1 = (uint32_t)(seed_ >> 32);
+        a.s0 = (uint32_t)s;
+        a.s1 = (uint32_t)(s >> 32);
+    }
+    naiveSweep<real>(a, dev_->stream());
+    ++stepsSinceRefresh_;
+    clearState(solSolutionAvailable);
+}
+
+template <class real>
 void DenseGraphAnnealer<real>::stepDone() {
     abortPending_ = true; /* the sweep reports an abort in h_abort_ itself */
     ++stepsSinceRefresh_;
@@ -729,6 +800,12 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     throwErrorIfQNotSet();
     ensureDevice();
     checkAbort(false);
+    if (algo_ == algoNaive) {
+        annealOneStepNaive(G, beta);
+        if (sysTraceCap_ > 0)
+            traceSystemE(G, beta);
+        return;
+    }
     PhaseEvents pe;
     if (refreshDue()) {
         if (profile_ >= 2)
@@ -768,6 +845,8 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     if (pending_.size() > 3000)
         collectPhases();
     stepDone();
+    if (sysTraceCap_ > 0)
+        traceSystemE(G, beta);
 }
 
 template <class real>
@@ -950,6 +1029,58 @@ real DenseGraphAnnealer<real>::getSystemE(real G, real beta) {
     if (om_ == optMaximize)
         E *= real(-1.);
     return E;
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::setSystemETrace(int capacity) {
+    SQ_THROW_IF(capacity < 0, "trace capacity must be >= 0");
+    ensureDevice();
+    sysTraceCap_ = capacity;
+    sysTraceLen_ = 0;
+    sysTraceSteps_ = 0;
+    if (capacity > 0)
+        d_sysTrace_.allocate((size_t)capacity);
+    else
+        d_sysTrace_.release();
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::traceSystemE(real G, real beta) {
+    /* what getSystemE(G, beta) returns for the state after this step
+     * (CPUDenseGraphAnnealer.cpp:230-247), computed on the device into the
+     * trace with no host synchronisation */
+    ++sysTraceSteps_;
+    if (sysTraceLen_ >= sysTraceCap_)
+        return;
+    energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
+                     dev_->stream());
+    const int sqa = isSQAAlgorithm(algo_) ? 1 : 0;
+    real coef = real(0.);
+    if (sqa) {
+        spinDotSum(d_Q_.ptr, N_, m_, Nw_, d_spinDot_.ptr, dev_->stream());
+        coef = real(0.5) / beta * std::log(std::tanh(G * beta / real(m_)));
+    }
+    const real sgn = (om_ == optMaximize) ? real(-1.) : real(1.);
+    systemEnergy<real>(d_E_.ptr, m_, sgn, d_spinDot_.ptr, sqa, coef, sgn, d_sysTrace_.ptr + sysTraceLen_,
+                       dev_->stream());
+    ++sysTraceLen_;
+}
+
+template <class real>
+int DenseGraphAnnealer<real>::getSystemETrace(double *out, int cap, long long *steps) {
+    ensureDevice();
+    const int n = std::min(cap, sysTraceLen_);
+    std::vector<real> v((size_t)std::max(n, 0));
+    if (n > 0) {
+        HIP_CHECK(hipMemcpyAsync(v.data(), d_sysTrace_.ptr, (size_t)n * sizeof(real), hipMemcpyDeviceToHost,
+                                 dev_->stream()));
+        dev_->synchronize();
+    }
+    for (int i = 0; i < n; ++i)
+        out[i] = (double)v[i];
+    if (steps)
+        *steps = sysTraceSteps_;
+    return n;
 }
 
 template <class real>

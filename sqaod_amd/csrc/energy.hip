@@ -162,6 +162,40 @@ template void energyBipartiteFP64<double>(const double *, const double *, const 
 
 }  // namespace sqaod_rocm
 
+/* ---------------- system energy (dense getSystemE) ---------------- */
+
+namespace sqaod_rocm {
+
+namespace {
+
+template <class real>
+__global__ void systemEnergyKernel(const double *E, int m, real sgnE, const double *spinDot, int sqa, real coef,
+                                   real sgnOut, real *out) {
+    real s = (real)0.;
+    for (int y = 0; y < m; ++y)
+        s += sgnE * (real)E[y];
+    s = s / (real)m;
+    if (sqa)
+        s -= (real)spinDot[0] * coef;
+    out[0] = s * sgnOut;
+}
+
+}  // namespace
+
+template <class real>
+void systemEnergy(const double *d_E, int m, real sgnE, const double *d_spinDot, int sqa, real coef, real sgnOut,
+                  real *d_out, hipStream_t stream) {
+    systemEnergyKernel<real><<<1, 1, 0, stream>>>(d_E, m, sgnE, d_spinDot, sqa, coef, sgnOut, d_out);
+    HIP_CHECK(hipGetLastError());
+}
+
+template void systemEnergy<float>(const double *, int, float, const double *, int, float, float, float *,
+                                  hipStream_t);
+template void systemEnergy<double>(const double *, int, double, const double *, int, double, double, double *,
+                                   hipStream_t);
+
+}  // namespace sqaod_rocm
+
 /* ---------------- batched bilinear forms (the formulas objects) ---------------- */
 
 namespace sqaod_rocm {

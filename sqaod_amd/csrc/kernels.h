@@ -39,6 +39,17 @@ template <class real>
 void streamToAttempts(const uint32_t *d_words, int N, int m, int order, uint32_t *d_X, real *d_U,
                       hipStream_t stream);
 
+/* sqaod.cpu's OpenMP coloring step with T workers
+ * (CPUDenseGraphAnnealer.cpp:303-339): worker w draws from its own generator
+ * d_states[w] (seeded s + 17 w, :29-33) for the slices of each colour in its
+ * static chunk, worker 0 also for slice m-1 of odd m between the colours.
+ * Generates every worker's words of one step (worker w at
+ * d_words + w * mtWorkerStride) and writes X, U in (slice, attempt) order. */
+unsigned long long mtWorkerStride(int N, int m, int T, int perAttempt);
+template <class real>
+void workerStreamsToAttempts(MTState *d_states, uint32_t *d_words, int N, int m, int T, uint32_t *d_X, real *d_U,
+                             hipStream_t stream);
+
 /* Counter-based stream (Philox4x32-10): attempt (y, k) of sweep `sweepIdx`
  * gets one Philox block keyed by the seed. */
 template <class real>
@@ -204,6 +215,23 @@ void bfSearchRange(const double *d_W, const double *d_EL, const BFPlan &p, unsig
                    unsigned *d_blockCnt, unsigned long long *d_offsets, int maxBlocks,
                    unsigned long long *d_xList, hipStream_t stream);
 
+/* ---------------- dense naive step (dense_naive.hip) ---------------- */
+
+template <class real>
+struct NaiveArgs {
+    int N, Np, m, Nw;
+    real twoDivM, coef, beta;          /* host scalars of annealOneStepNaive (:272-273) */
+    const real *J;                     /* N x Np rows the fields are updated with */
+    const real *h;                     /* Np */
+    real *F;                           /* m x Np local fields, in/out */
+    uint32_t *Qbits;                   /* m x Nw spins, in/out */
+    const uint32_t *words;             /* MT19937 words (3 per float attempt, 4 per double), or null: Philox */
+    uint32_t k0, k1, s0, s1;           /* Philox key (seed) and sweep counter */
+    unsigned long long *counters;      /* [0] accepted flips, [1] batches */
+};
+template <class real>
+void naiveSweep(const NaiveArgs<real> &a, hipStream_t stream);
+
 /* ---------------- dense sweep (dense_sweep.hip) ---------------- */
 
 template <class real>
@@ -322,6 +350,15 @@ void quadForm(const QuadFormArgs<real> &a, int nOut, hipStream_t stream);
 
 /* sum_y q_y . q_(y+1)%m (the Trotter coupling of getSystemE), in double. */
 void spinDotSum(const uint32_t *d_Qbits, int N, int m, int Nw, double *d_out, hipStream_t stream);
+
+/* getSystemE of the dense annealer (CPUDenseGraphAnnealer.cpp:230-247) on
+ * the device, from the FP64 energies d_E (m) and the slice overlap sum
+ * d_spinDot, in real arithmetic in the host formula's order:
+ * E = (sum_y sgnE (real)E_y) / m, then E -= (real)spinDot * coef when sqa,
+ * then E *= sgnOut; written to *d_out. */
+template <class real>
+void systemEnergy(const double *d_E, int m, real sgnE, const double *d_spinDot, int sqa, real coef, real sgnOut,
+                  real *d_out, hipStream_t stream);
 
 /* ---------------- self checks (diag.hip) ---------------- */
 

@@ -39,7 +39,8 @@ __device__ __forceinline__ uint32_t mtTemper(uint32_t y) {
     return y;
 }
 
-__global__ __launch_bounds__(1024) void mtGenerateKernel(MTState *st, uint32_t *out, unsigned long long n) {
+/* one workgroup of 1024 threads continues st by n words into out */
+__device__ void mtGenerateBlock(MTState *st, uint32_t *out, unsigned long long n) {
     __shared__ uint32_t mt[624];
     __shared__ uint32_t nw[624];
     const int t = threadIdx.x;
@@ -82,6 +83,36 @@ __global__ __launch_bounds__(1024) void mtGenerateKernel(MTState *st, uint32_t *
         st->idx = idx;
 }
 
+__global__ __launch_bounds__(1024) void mtGenerateKernel(MTState *st, uint32_t *out, unsigned long long n) {
+    mtGenerateBlock(st, out, n);
+}
+
+/* Slices of one colour handled by worker w of T (libgomp's schedule(static)
+ * without a chunk size: contiguous chunks, the first nHalf % T workers one
+ * larger); *b = its first slice index within the colour. */
+__device__ __forceinline__ int workerChunk(int nHalf, int T, int w, int *b) {
+    const int q = nHalf / T, r = nHalf % T;
+    *b = w * q + (w < r ? w : r);
+    return q + (w < r ? 1 : 0);
+}
+
+/* Attempts worker w makes per iteration of the coloring step: its even
+ * slices, its odd slices, and for odd m worker 0 also slice m-1
+ * (CPUDenseGraphAnnealer.cpp:303-327). */
+__device__ __forceinline__ int workerAttemptsPerIt(int nHalf, int T, int oddM, int w) {
+    int b;
+    return 2 * workerChunk(nHalf, T, w, &b) + ((w == 0 && oddM) ? 1 : 0);
+}
+
+/* block w: worker w's words of one coloring step, from its own generator */
+__global__ __launch_bounds__(1024) void mtGenerateWorkersKernel(MTState *st, uint32_t *out, unsigned long long stride,
+                                                                int T, int nHalf, int oddM, int N, int perAttempt) {
+    const int w = blockIdx.x;
+    const unsigned long long n =
+        (unsigned long long)N * (unsigned long long)workerAttemptsPerIt(nHalf, T, oddM, w) * perAttempt;
+    mtGenerateBlock(st + w, out + (unsigned long long)w * stride, n);
+}
+
 template <class real>
 struct Draws;
 template <>
@@ -103,6 +134,34 @@ __device__ __forceinline__ long long attemptPosition(int order, int N, int m, in
     int nEven = (m + 1) / 2;
     int pos = (y & 1) ? nEven + (y >> 1) : (y >> 1);
     return (long long)k * m + pos;
+}
+
+/* Attempt (y, k) of a coloring step in sqaod.cpu's OpenMP order: the
+ * worker owning slice y draws it at position k * (its attempts per
+ * iteration) + (its position within the iteration: even chunk, then slice
+ * m-1 for worker 0 and odd m, then odd chunk). */
+template <class real>
+__global__ void workerStreamsToAttemptsKernel(const uint32_t *words, unsigned long long stride, int N, int m, int T,
+                                              uint32_t *X, real *U) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)N * m)
+        return;
+    const int y = (int)(idx / N), k = (int)(idx % N);
+    const int nHalf = m / 2, oddM = m & 1;
+    int w, j, b;
+    if (oddM && y == m - 1) {
+        w = 0;
+        j = workerChunk(nHalf, T, 0, &b);
+    } else {
+        const int kk = y >> 1, q = nHalf / T, r = nHalf % T;
+        w = (kk < r * (q + 1)) ? kk / (q + 1) : r + (kk - r * (q + 1)) / q;
+        const int c = workerChunk(nHalf, T, w, &b);
+        j = (kk - b) + ((y & 1) ? c + ((w == 0 && oddM) ? 1 : 0) : 0);
+    }
+    const long long p = (long long)k * workerAttemptsPerIt(nHalf, T, oddM, w) + j;
+    const uint32_t *wd = words + (unsigned long long)w * stride + (unsigned long long)p * Draws<real>::perAttempt;
+    X[idx] = wd[0] % (uint32_t)N;
+    U[idx] = Draws<real>::uniform(wd);
 }
 
 template <class real>
@@ -230,6 +289,28 @@ void mtGenerate(MTState *d_state, uint32_t *d_out, size_t n, hipStream_t stream)
     mtGenerateKernel<<<1, 1024, 0, stream>>>(d_state, d_out, (unsigned long long)n);
     HIP_CHECK(hipGetLastError());
 }
+
+unsigned long long mtWorkerStride(int N, int m, int T, int perAttempt) {
+    const int nHalf = m / 2;
+    const int cMax = nHalf / T + (nHalf % T ? 1 : 0);
+    return (unsigned long long)N * (unsigned long long)(2 * cMax + 1) * (unsigned long long)perAttempt;
+}
+
+template <class real>
+void workerStreamsToAttempts(MTState *d_states, uint32_t *d_words, int N, int m, int T, uint32_t *d_X, real *d_U,
+                             hipStream_t stream) {
+    SQ_THROW_IF(T < 1 || T > 1024, "worker count %d out of range", T);
+    const int per = sizeof(real) == 4 ? 2 : 3;
+    const unsigned long long stride = mtWorkerStride(N, m, T, per);
+    mtGenerateWorkersKernel<<<T, 1024, 0, stream>>>(d_states, d_words, stride, T, m / 2, m & 1, N, per);
+    HIP_CHECK(hipGetLastError());
+    long long total = (long long)N * m;
+    workerStreamsToAttemptsKernel<real><<<gridFor(total, 256), 256, 0, stream>>>(d_words, stride, N, m, T, d_X, d_U);
+    HIP_CHECK(hipGetLastError());
+}
+template void workerStreamsToAttempts<float>(MTState *, uint32_t *, int, int, int, uint32_t *, float *, hipStream_t);
+template void workerStreamsToAttempts<double>(MTState *, uint32_t *, int, int, int, uint32_t *, double *,
+                                              hipStream_t);
 
 template <class real>
 void streamToAttempts(const uint32_t *d_words, int N, int m, int order, uint32_t *d_X, real *d_U, hipStream_t stream) {

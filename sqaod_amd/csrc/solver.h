@@ -155,6 +155,13 @@ public:
      * in FP64: E_form uses the J the field form anneals (bf16 hi + lo, or the
      * fp8 J'), E the J as given; 0 for the fp32/fp64 form. */
     double formEnergyError();
+    /* System-energy trace (no reference counterpart): with capacity > 0,
+     * every annealOneStep(G, beta) appends getSystemE(G, beta) of the state
+     * after the step, computed on the device without a host synchronisation;
+     * the first `capacity` steps are kept.  0 turns it off.  Reading copies
+     * the kept values (one synchronisation) and the number of steps traced. */
+    void setSystemETrace(int capacity);
+    int getSystemETrace(double *out, int cap, long long *steps);
 
     /* Hooks of the batch engine (dense_batch.cpp), which runs many
      * instances' steps in shared launches with the same kernels and the same
@@ -190,7 +197,10 @@ private:
     void ensureDevice();
     void checkAbort(bool synchronous);
     void allocateProblem(int N);
-    void refreshFields();
+    void refreshFields(bool always = false); /* always: also when the plain sweep needs no fields */
+    void annealOneStepNaive(real G, real beta);
+    void seedWorkerGenerators(); /* d_mt_[w] = MT19937 seeded seed_ + 17 w, w < nWorkers_ */
+    void traceSystemE(real G, real beta);
     void generateAttempts();
     void uploadJ(const real *J, const real *h, real c);
     void prepareFieldForm(); /* derives the form's J (bf16 split / fp8 quantization) from d_J_ */
@@ -214,6 +224,10 @@ private:
     int refreshInterval_ = 1;     /* "refresh_interval": steps between F recomputes (0: only when spins were set) */
     bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
     int spinLimit_ = 1 << 20;
+    int nWorkers_ = 1;        /* CPU workers whose MT19937 streams the coloring step replays */
+    int sysTraceCap_ = 0, sysTraceLen_ = 0;
+    long long sysTraceSteps_ = 0;
+    int mtWorkersSeeded_ = 0; /* generators in d_mt_ */
 
     int profile_ = 0;             /* "profile": 1 = HIP events around each sweep, 2 = around every phase */
     struct PhaseEvents {
@@ -247,6 +261,7 @@ private:
     DevBuf<uint32_t> d_Q_, d_X_, d_D_, d_words_;
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
+    DevBuf<real> d_sysTrace_;
     DevBuf<int> d_abort_;
     DevBuf<unsigned long long> d_counters_;
     DevBuf<int8_t> d_q8_;

@@ -55,6 +55,9 @@ _SIGS = {
     "sqaod_rocm_dg_annealer_get_counters": (_i32, [_vp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
     "sqaod_rocm_dg_annealer_get_form_energy_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dg_annealer_set_system_E_trace": (_i32, [_vp, _i32]),
+    "sqaod_rocm_dg_annealer_get_system_E_trace": (_i32, [_vp, _vp, _i32, ctypes.POINTER(_i32),
+                                                         ctypes.POINTER(ctypes.c_longlong)]),
     "sqaod_rocm_dg_annealer_get_fields": (_i32, [_vp, _vp, _i32, _i32]),
     "sqaod_rocm_dg_annealer_get_phase_times": (_i32, [_vp, ctypes.POINTER(_f64), ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_bf_searcher_new": (_i32, [_i32, _PP]),
@@ -182,7 +185,8 @@ def _dt(dtype):
     raise RuntimeError("dtype must be np.float32 or np.float64")
 
 
-_INT_PREFS = ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval', 'profile')
+_INT_PREFS = ('n_trotters', 'tile_size', 'slices_per_block', 'refresh_interval', 'profile', 'n_workers',
+              'spin_limit')
 
 
 def _set_prefs(setter, cobj, prefs):
@@ -389,6 +393,18 @@ class _DenseGraphAnnealer:
         cnt = (ctypes.c_ulonglong * 3)()
         _check(_lib.sqaod_rocm_dg_annealer_get_phase_times(cobj, ms, cnt))
         return {name: (ms[i], cnt[i]) for i, name in enumerate(("refresh", "rng", "sweep"))}
+
+    @staticmethod
+    def set_system_E_trace(cobj, capacity, dtype):
+        _check(_lib.sqaod_rocm_dg_annealer_set_system_E_trace(cobj, int(capacity)))
+
+    @staticmethod
+    def get_system_E_trace(cobj, cap, dtype):
+        out = np.empty(max(int(cap), 1), np.float64)
+        n, steps = _i32(), ctypes.c_longlong()
+        _check(_lib.sqaod_rocm_dg_annealer_get_system_E_trace(cobj, _ptr(out), int(cap), ctypes.byref(n),
+                                                              ctypes.byref(steps)))
+        return out[:n.value], steps.value
 
     @staticmethod
     def get_energy_rounding_error(cobj, dtype):

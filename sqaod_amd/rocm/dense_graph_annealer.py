@@ -30,6 +30,17 @@ class DenseGraphAnnealer(DenseGraphAnnealerBase):
         """Local fields F = J q_y (m x N) of the local-field sweep."""
         return cext.get_fields(self._cobj, self.dtype)
 
+    def trace_system_E(self, capacity):
+        """From now on every anneal_one_step(G, beta) records get_system_E(G,
+        beta) of the state after the step, on the device and without a host
+        synchronisation (the first `capacity` steps); 0 turns it off."""
+        cext.set_system_E_trace(self._cobj, capacity, self.dtype)
+        self._trace_cap = int(capacity)
+
+    def get_system_E_trace(self):
+        """The recorded values (float64 array) and the number of steps traced."""
+        return cext.get_system_E_trace(self._cobj, getattr(self, "_trace_cap", 0), self.dtype)
+
     def get_energy_rounding_error(self):
         """max |E(dtype) - E(fp64)| of the last energy evaluation."""
         return cext.get_energy_rounding_error(self._cobj, self.dtype)

@@ -70,16 +70,18 @@ def test_state_machine_errors_without_gpu(cext):
 
 
 def test_preferences_defaults_and_algorithm_fallback(cext):
-    """Default algorithm coloring; naive maps to coloring and sa_default to
-    sa_naive, as the CUDA annealer does (CUDADenseGraphAnnealer.cu:115-126,
-    test_dense_graph_annealer.py:454-471)."""
+    """Default algorithm coloring; naive is offered, as on the CPU annealer
+    (CPUDenseGraphAnnealer.cpp:35-50, test_dense_graph_annealer.py:277-290);
+    sa_default and sa_coloring map to sa_naive (Solver.cpp:152-168)."""
     A = cext.rocm_dg_annealer
     h = A.new(np.float64)
     try:
         p = A.get_preferences(h, np.float64)
         assert p["algorithm"] == "coloring" and p["device"] == "rocm" and p["precision"] == "double"
         A.set_preferences(h, {"algorithm": "naive"}, np.float64)
-        assert A.get_preferences(h, np.float64)["algorithm"] == "coloring"
+        assert A.get_preferences(h, np.float64)["algorithm"] == "naive"
+        A.set_preferences(h, {"algorithm": "sa_coloring"}, np.float64)
+        assert A.get_preferences(h, np.float64)["algorithm"] == "sa_naive"
         A.set_preferences(h, {"algorithm": "sa_default"}, np.float64)
         assert A.get_preferences(h, np.float64)["algorithm"] == "sa_naive"
         A.set_preferences(h, {"algorithm": "default", "n_trotters": 7}, np.float64)

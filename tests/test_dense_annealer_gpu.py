@@ -257,3 +257,54 @@ def test_spin_limit_gives_up_cleanly(rocm):
     an.make_solution()
     x = np.stack(an.get_x()).astype(np.float64)
     assert np.array_equal(an.get_E(), np.einsum("yi,yi->y", x @ W.astype(np.float64), x).astype(np.float32))
+
+
+@pytest.mark.parametrize("T,N,m", [(8, 128, 32), (3, 128, 32), (16, 100, 7), (4, 64, 5), (2, 50, 2)])
+def test_openmp_worker_streams_bit_exact(oracle, rocm, T, N, m):
+    """sqaod.cpu's default on a multi-core host is the OpenMP coloring step
+    (CPUDenseGraphAnnealer.cpp:303-339) with one MT19937 per worker seeded
+    s + 17 w (:29-33) and the slices of each colour in static chunks; with
+    n_workers = T the device replays that trajectory (the oracle's
+    restatement with T workers): spins after every 10th step of the
+    schedule, and the energies, bit for bit.  Odd m puts slice m-1 on worker
+    0 between the colours; T above m/2 leaves workers without slices."""
+    W = dense_int_W(N, SEED_BASE + 90 + T + m)
+    ora = oracle.DenseGraphAnnealer(np.float32, T)
+    ora.seed(1000)
+    ora.set_qubo(W)
+    ora.set_m(m)
+    ora.prepare()
+    ora.randomize_spin()
+    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m, rng="mt19937", n_workers=T)
+    assert an.get_preferences()["n_workers"] == T
+    an.seed(1000)
+    an.prepare()
+    an.randomize_spin()
+    assert np.array_equal(q_of(an), ora.get_q())
+    for k, G in enumerate(schedule()[:100]):
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+        if k % 10 == 0:
+            assert np.array_equal(q_of(an), ora.get_q()), f"step {k}"
+    assert np.array_equal(q_of(an), ora.get_q())
+    assert np.array_equal(an.get_E(), ora.calculate_E())
+
+
+def test_generators_continue_across_prepare(oracle, rocm):
+    """prepare() called again does not restart the random streams (the CPU
+    annealer seeds only in seed(), CPUDenseGraphAnnealer.cpp:29-33, 164-167):
+    the reference tests' calling sequence prepare / randomize_spin / anneal,
+    twice (test_dense_graph_annealer.py:356-361), stays bit-exact."""
+    N, m = 60, 6
+    W = dense_int_W(N, SEED_BASE + 97)
+    ora, an = make_pair(oracle, rocm, W, m, seed=4)
+    for rep in range(2):
+        if rep:
+            ora.prepare()
+            ora.randomize_spin()
+            an.prepare()
+            an.randomize_spin()
+        for G in schedule()[:15]:
+            ora.anneal_one_step(G, BETA)
+            an.anneal_one_step(G, BETA)
+        assert np.array_equal(q_of(an), ora.get_q()), rep
