@@ -178,6 +178,23 @@ int sqaod_rocm_dg_bf_searcher_make_solution(sqaod_rocm_dg_bf_searcher *b);
 int sqaod_rocm_dg_bf_searcher_search_range(sqaod_rocm_dg_bf_searcher *b, int *done, unsigned long long *cur_x_end);
 /* bf_searcher_search (:456-473): prepare, scan everything, make_solution */
 int sqaod_rocm_dg_bf_searcher_search(sqaod_rocm_dg_bf_searcher *b);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the search of sqaod_rocm_dg_bf_searcher_search (same
+ * kernels, same minimum and x list) on the caller's device buffers, with
+ * every intermediate in a static 256 KiB device array of the library (one
+ * per GPU, held until the process ends); no allocation or free.
+ *   d_W: N x N symmetric QUBO of dtype (0 float32, 1 float64) in device
+ *   memory, N in 1..63; optimize 0 = minimize, 1 = maximize;
+ *   tile_size in 1..2^26, rounded up to a multiple of 256 and capped at 2^N
+ *   as the searcher does -- d_x must hold that many entries;
+ *   d_Emin: one value of dtype; d_x: the packed minimisers in ascending order
+ *   (bit N-1-i = x[i]); d_n: their number (unsigned long long);
+ *   stream: the hipStream_t to enqueue on (NULL: the null stream).
+ * Asynchronous: the results are ready when the stream reaches them.  Calls
+ * that overlap on one GPU share the scratch array and race. */
+int sqaod_rocm_dense_bf_search_noalloc(int dtype, const void *d_W, int N, int optimize, unsigned long long tile_size,
+                                       void *d_Emin, unsigned long long *d_x, unsigned long long *d_n, void *stream);
 /* number of solutions (rows of get_x); get_E has max(1, n) entries */
 int sqaod_rocm_dg_bf_searcher_get_n_solutions(sqaod_rocm_dg_bf_searcher *b, int *n);
 /* bf_searcher_get_x (:241-275): x[n x N] int8 */

@@ -772,6 +772,12 @@ int sqaod_rocm_dg_bf_searcher_search(sqaod_rocm_dg_bf_searcher *b) {
     return guarded([&] { withBf(b, [&](auto &s) { s.search(); }); });
 }
 
+int sqaod_rocm_dense_bf_search_noalloc(int dtype, const void *d_W, int N, int optimize, unsigned long long tile_size,
+                                       void *d_Emin, unsigned long long *d_x, unsigned long long *d_n, void *stream) {
+    return guarded(
+        [&] { denseBFSearchNoalloc(dtype, d_W, N, optimize, tile_size, d_Emin, d_x, d_n, (hipStream_t)stream); });
+}
+
 int sqaod_rocm_dg_bf_searcher_get_n_solutions(sqaod_rocm_dg_bf_searcher *b, int *n) {
     return guarded([&] {
         SQ_THROW_IF(!n, "null output pointer");

@@ -215,6 +215,13 @@ void bfSearchRange(const double *d_W, const double *d_EL, const BFPlan &p, unsig
                    unsigned *d_blockCnt, unsigned long long *d_offsets, int maxBlocks,
                    unsigned long long *d_xList, hipStream_t stream);
 
+/* The dense search with the caller's device buffers and a static scratch
+ * array (bf_noalloc.hip): W real N x N, results Emin (real, sign of the
+ * optimize direction restored), the ascending x list (capacity:
+ * min(tileSize rounded up to 256, 2^N)) and its length. */
+void denseBFSearchNoalloc(int dtype, const void *d_W, int N, int optimize, unsigned long long tileSize, void *d_Emin,
+                          unsigned long long *d_xList, unsigned long long *d_n, hipStream_t stream);
+
 /* ---------------- dense naive step (dense_naive.hip) ---------------- */
 
 template <class real>

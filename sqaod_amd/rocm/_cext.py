@@ -72,6 +72,7 @@ _SIGS = {
     "sqaod_rocm_dg_bf_searcher_make_solution": (_i32, [_vp]),
     "sqaod_rocm_dg_bf_searcher_search_range": (_i32, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_u64)]),
     "sqaod_rocm_dg_bf_searcher_search": (_i32, [_vp]),
+    "sqaod_rocm_dense_bf_search_noalloc": (_i32, [_i32, _vp, _i32, _i32, _u64, _vp, _vp, _vp, _vp]),
     "sqaod_rocm_dg_bf_searcher_get_n_solutions": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "sqaod_rocm_dg_bf_searcher_get_x": (_i32, [_vp, _vp, _i32, _i32]),
     "sqaod_rocm_dg_bf_searcher_get_E": (_i32, [_vp, _vp, _i32]),
