@@ -15,7 +15,8 @@ with annealer seeds 1000 .. 1999, one anneal per seed: seed, randomize_spin,
 205 steps, then E of every slice (CPUBipartiteGraphAnnealer.cpp:215-222).
 Each seed contributes min_y E_y and mean_y E_y.
 
-Runs for about ten minutes on 8 cores:  python tests/golden/make_bipartite_ks.py
+About 16 s per seed on one core (40 minutes with 6 worker processes):
+    KS_PROCS=6 python tests/golden/make_bipartite_ks.py
 """
 import json
 import os
