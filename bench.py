@@ -93,9 +93,11 @@ def bytes_per_attempt(N, form):
     return N * bJ + N * 1 + 8
 
 
-def run_form(rocm, W, m, form, steps, warmup, seed):
+def run_form(rocm, W, m, form, steps, warmup, seed, multi_flip=None):
     N = W.shape[0]
     prefs = dict(n_trotters=m, profile=1)
+    if multi_flip is not None and form in ("fp32", "bf16x2"):
+        prefs.update(multi_flip=multi_flip)
     if form == "plain":
         prefs.update(sweep="plain")
     else:
@@ -123,6 +125,13 @@ def run_form(rocm, W, m, form, steps, warmup, seed):
     sweep_ms, sweep_n = phases["sweep"]
     avg_sweep_ms = sweep_ms / max(sweep_n, 1)
     E = an.get_E()
+    # the energy evaluation (calculate_E: integer-plane MFMA kernel, final
+    # sums, copy of m doubles and a synchronisation), host wall time per call
+    n_e = 10
+    te = time.perf_counter()
+    for _ in range(n_e):
+        an.calculate_E()
+    energy_call_ms = 1e3 * (time.perf_counter() - te) / n_e
     Eerr = an.get_energy_rounding_error()
     Eform = an.get_form_energy_error()
     flips = c1["flips"] - c0["flips"]
@@ -139,6 +148,7 @@ def run_form(rocm, W, m, form, steps, warmup, seed):
         "E_form_max_abs_err": Eform,
         "E_form_max_rel_err": Eform / max(abs(float(np.min(E))), 1e-30),
         "flips_per_launch": flips / max(sweep_n, 1),
+        "energy_call_ms": energy_call_ms,
     }
     if form != "plain":
         # the field GEMM's own time: HIP events around every phase, in a pass
@@ -396,6 +406,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--multi-flip", choices=("on", "off"), default=None,
+                    help="multi-flip batches of the float local-field forms (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the bipartite, BF search and serving runs")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--batch-instances", type=int, default=512)
@@ -416,7 +428,7 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     for form in forms:
-        results[form] = run_form(rocm, W, m, form, args.steps, args.warmup, seed=1000)
+        results[form] = run_form(rocm, W, m, form, args.steps, args.warmup, seed=1000, multi_flip=args.multi_flip)
         print(f"# {form}: {results[form]['attempts_per_s']:.4g} attempts/s, "
               f"{results[form]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
 
@@ -487,6 +499,12 @@ def main():
                     f"{'fp8' if head == 'fp8' else 'bf16'} MFMA ({planes} J planes); time from HIP events around "
                     f"the refresh in 10 steps after the timed region",
         }
+    out["energy"] = {
+        "kernel": "planeEnergyKernel + planeFinalKernel (E of all m slices, J as int8 digit planes on MFMA)",
+        "call_ms": r["energy_call_ms"],
+        "note": "host wall time per calculate_E, including its device-to-host copy of m doubles and the "
+                "stream synchronisation; the kernels' own durations are in the rocprofv3 summary",
+    }
     if "plain" in results and head in results:
         out["speedup_vs_plain_fp32"] = r["attempts_per_s"] / results["plain"]["attempts_per_s"]
     if "fp32" in results and head in results:

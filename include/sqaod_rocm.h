@@ -79,7 +79,12 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
  * "algorithm" ("coloring" | "naive" | "sa_naive" | "default" | "sa_default";
  * others fall back to the default of their kind, Solver.cpp:152-168),
  * "n_trotters" (int), and the rocm-device knobs "rng" ("philox" | "mt19937"),
+ * "n_workers" (int >= 1: with rng = mt19937, the per-worker streams of the
+ * CPU annealer's OpenMP coloring step with that many threads),
  * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2" | "fp8"),
+ * "multi_flip" ("off" | "on": float local-field sweeps commit every flip of a
+ * 64-attempt batch in order instead of ending the batch at its first flip;
+ * same results, faster only while many attempts are accepted),
  * "slices_per_block" (int, 0 = auto), "sweep_launch" ("auto" | "cooperative"),
  * "refresh_interval" (int, steps between
  * field recomputes; 0 = only after the spins were set), "refresh_engine"
@@ -122,6 +127,11 @@ int sqaod_rocm_dg_annealer_anneal_one_step(sqaod_rocm_dg_annealer *a, double G, 
  * max |E_dtype - E_fp64| of the last energy evaluation. */
 int sqaod_rocm_dg_annealer_get_counters(sqaod_rocm_dg_annealer *a, unsigned long long counters[3]);
 int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, double *err);
+/* Diagnostic without a reference counterpart: E[m] of the last energy
+ * evaluation (calculate_E / make_solution) in double, before its rounding to
+ * dtype.  Exact on integer problems; otherwise within FP64 rounding of the
+ * exact value for the J held in dtype. */
+int sqaod_rocm_dg_annealer_get_E_fp64(sqaod_rocm_dg_annealer *a, double *E, int m);
 /* System-energy trace, for callers that record getSystemE after every step
  * (sqaodpy/sqaod/wildqat/opt.py:33-35, 57-62 calls get_system_E per step);
  * no reference counterpart.  With capacity > 0, every anneal_one_step(G,
@@ -260,6 +270,9 @@ int sqaod_rocm_bg_annealer_get_preferences(sqaod_rocm_bg_annealer *a, char *buf,
 int sqaod_rocm_bg_annealer_get_n_trotters(sqaod_rocm_bg_annealer *a, int *m);
 /* annealer_get_E (:332-364): E[m] of dtype */
 int sqaod_rocm_bg_annealer_get_E(sqaod_rocm_bg_annealer *a, void *E, int m);
+/* Diagnostic without a reference counterpart: E[m] of the last energy
+ * evaluation in double, before its rounding to dtype. */
+int sqaod_rocm_bg_annealer_get_E_fp64(sqaod_rocm_bg_annealer *a, double *E, int m);
 /* annealer_get_x (:367-392): x0[m x N0], x1[m x N1] int8 */
 int sqaod_rocm_bg_annealer_get_x(sqaod_rocm_bg_annealer *a, int8_t *x0, int8_t *x1, int m, int N0, int N1);
 /* annealer_get_hamiltonian (:470-503): h0[N0], h1[N1], J[N1 x N0], c[1] of dtype */

@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libsqaod_rocm.so")
 ARCH = os.environ.get("SQAOD_ROCM_ARCH", "gfx950")
 
-SOURCES = ["rng.hip", "dense_sweep.hip", "dense_naive.hip", "dense_field.hip", "dense_field_mfma.hip", "energy.hip", "dense_bf.hip", "bf_noalloc.hip", "bipartite.hip", "bipartite_bf.hip", "diag.hip", "dense_annealer.cpp", "dense_bf_searcher.cpp", "bipartite_annealer.cpp", "formulas.cpp", "bipartite_bf_searcher.cpp", "dense_batch.cpp", "serve.cpp", "capi.cpp"]
+SOURCES = ["rng.hip", "dense_sweep.hip", "dense_naive.hip", "dense_field.hip", "dense_field_mfma.hip", "energy.hip", "energy_planes.hip", "dense_bf.hip", "bf_noalloc.hip", "bipartite.hip", "bipartite_bf.hip", "diag.hip", "dense_annealer.cpp", "dense_bf_searcher.cpp", "bipartite_annealer.cpp", "formulas.cpp", "bipartite_bf_searcher.cpp", "dense_batch.cpp", "serve.cpp", "capi.cpp"]
 
 # -ffp-contract=off: the annealers' float arithmetic must round exactly like
 # the CPU annealer (no fused multiply-adds); the packed-FP32 feature is off.

@@ -161,6 +161,7 @@ void BipartiteGraphAnnealer<real>::uploadProblem(const std::vector<double> &h0, 
     HIP_CHECK(hipMemcpyAsync(d_h1_.ptr, h1p.data(), h1p.size() * sizeof(real), hipMemcpyHostToDevice, dev_->stream()));
     dev_->synchronize();
     fragValid_ = false;
+    planesValid_ = false;
 }
 
 template <class real>
@@ -256,7 +257,7 @@ void BipartiteGraphAnnealer<real>::prepare() {
     HIP_CHECK(hipMemsetAsync(d_Q1_.ptr, 0, d_Q1_.bytes(), dev_->stream()));
     d_dE_.allocate((size_t)m_ * std::max(N0p_, N1p_));
     d_E_.allocate((size_t)m_);
-    d_partial_.allocate((size_t)m_ * ((N0p_ + N1p_) / 256));
+    ensureEnergyPlanes();
     d_spinDot_.allocate(2);
     if (rngMT_) {
         /* like the CPU's generator, the stream continues across prepare();
@@ -436,19 +437,54 @@ void BipartiteGraphAnnealer<real>::annealOneStep(real G, real beta) {
 }
 
 template <class real>
+void BipartiteGraphAnnealer<real>::ensureEnergyPlanes() {
+    /* J (N1 x N0, row stride N0p) as digit planes, rows for q1 and columns
+     * for q0; synchronises the stream when the planes are built */
+    if (!planesValid_) {
+        d_planeScan_.allocate(3);
+        planes_ = energyPlaneLayout<real>(d_J_.ptr, N0p_, 1, N1_, N0_, d_planeScan_.ptr, dev_->stream());
+        if (planes_.finite) {
+            d_planes_.allocate(energyPlaneFragCount(planes_));
+            energyPlaneBuild<real>(d_J_.ptr, N0p_, 1, planes_, d_planes_.ptr, dev_->stream());
+        } else {
+            d_planes_.release();
+        }
+        planesValid_ = true;
+    }
+    size_t parts = (size_t)m_ * ((N0p_ + N1p_) / 256); /* the FP64 vector kernel's */
+    if (planes_.finite)
+        parts = std::max(parts, energyPlanePartialCount(planes_, m_));
+    d_partial_.allocate(parts);
+}
+
+template <class real>
+void BipartiteGraphAnnealer<real>::computeEnergies() {
+    ensureEnergyPlanes();
+    if (planes_.finite)
+        energyPlanes<real>(planes_, d_planes_.ptr, d_h1_.ptr, d_h0_.ptr, (double)c_, d_Q0_.ptr, N0p_ / 32, d_Q1_.ptr,
+                           N1p_ / 32, m_, d_partial_.ptr, d_E_.ptr, dev_->stream());
+    else
+        energyBipartiteFP64<real>(d_JT_.ptr, d_h0_.ptr, d_h1_.ptr, (double)c_, d_Q0_.ptr, d_Q1_.ptr, N0_, N1_, N0p_,
+                                  N1p_, m_, d_partial_.ptr, d_E_.ptr, dev_->stream());
+}
+
+template <class real>
 void BipartiteGraphAnnealer<real>::calculate_E() {
-    /* :215-222 with BGFuncs::calculate_E (SharedFormulas.cpp:184-202), FP64 */
+    /* :215-222 with BGFuncs::calculate_E (SharedFormulas.cpp:184-202),
+     * evaluated exactly (integer planes) or to FP64 accuracy */
     throwErrorIfQNotSet();
     ensureDevice();
-    energyBipartiteFP64<real>(d_JT_.ptr, d_h0_.ptr, d_h1_.ptr, (double)c_, d_Q0_.ptr, d_Q1_.ptr, N0_, N1_, N0p_, N1p_,
-                              m_, d_partial_.ptr, d_E_.ptr, dev_->stream());
+    computeEnergies();
     std::vector<double> e((size_t)m_);
     HIP_CHECK(hipMemcpyAsync(e.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost, dev_->stream()));
     dev_->synchronize();
     const real sgn = (om_ == optMaximize) ? real(-1.) : real(1.);
     E_.resize((size_t)m_);
-    for (int y = 0; y < m_; ++y)
+    Efp64_.resize((size_t)m_);
+    for (int y = 0; y < m_; ++y) {
         E_[y] = sgn * (real)e[y];
+        Efp64_[y] = (double)sgn * e[y];
+    }
     setState(solEAvailable);
 }
 

@@ -419,6 +419,17 @@ int sqaod_rocm_dg_annealer_get_energy_rounding_error(sqaod_rocm_dg_annealer *a, 
     return guarded([&] { withDg(a, [&](auto &s) { *err = s.lastEnergyRoundingError(); }); });
 }
 
+int sqaod_rocm_dg_annealer_get_E_fp64(sqaod_rocm_dg_annealer *a, double *E, int m) {
+    return guarded([&] {
+        withDg(a, [&](auto &s) {
+            const std::vector<double> &e = s.energiesFP64();
+            SQ_THROW_IF(e.empty(), "get_E_fp64: no energies calculated yet");
+            SQ_THROW_IF(m != (int)e.size() || !E, "get_E_fp64: expected %d entries, got %d", (int)e.size(), m);
+            std::copy(e.begin(), e.end(), E);
+        });
+    });
+}
+
 int sqaod_rocm_dg_annealer_set_system_E_trace(sqaod_rocm_dg_annealer *a, int capacity) {
     return guarded([&] { withDg(a, [&](auto &s) { s.setSystemETrace(capacity); }); });
 }
@@ -1035,6 +1046,17 @@ int sqaod_rocm_bg_annealer_get_E(sqaod_rocm_bg_annealer *a, void *E, int m) {
             using real = typename std::remove_reference_t<decltype(s)>::real_type;
             SQ_THROW_IF(m != s.numTrotters(), "get_E: expected %d entries, got %d", s.numTrotters(), m);
             s.get_E(static_cast<real *>(E));
+        });
+    });
+}
+
+int sqaod_rocm_bg_annealer_get_E_fp64(sqaod_rocm_bg_annealer *a, double *E, int m) {
+    return guarded([&] {
+        withBg(a, [&](auto &s) {
+            const std::vector<double> &e = s.energiesFP64();
+            SQ_THROW_IF(e.empty(), "get_E_fp64: no energies calculated yet");
+            SQ_THROW_IF(m != (int)e.size() || !E, "get_E_fp64: expected %d entries, got %d", (int)e.size(), m);
+            std::copy(e.begin(), e.end(), E);
         });
     });
 }

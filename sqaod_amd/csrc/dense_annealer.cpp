@@ -375,6 +375,15 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
             nWorkers_ = v;
             clearState(solPrepared);
         }
+    } else if (name == "multi_flip") {
+        /* float local-field sweeps: batches that commit every flip of their
+         * 64 attempts in order ("on") or end at the first flip ("off") */
+        if (value == "on" || value == "1" || value == "true" || value == "True")
+            multiFlip_ = true;
+        else if (value == "off" || value == "0" || value == "false" || value == "False")
+            multiFlip_ = false;
+        else
+            SQ_THROW("multi_flip must be on or off");
     } else if (name == "spin_limit") {
         const int v = atoi(value.c_str());
         SQ_THROW_IF(v <= 0, "spin_limit must be positive");
@@ -407,6 +416,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
     p.emplace_back("refresh_engine", refreshVector_ ? "vector" : "mfma");
     p.emplace_back("profile", std::to_string(profile_));
+    p.emplace_back("multi_flip", multiFlip_ ? "on" : "off");
     return p;
 }
 
@@ -430,6 +440,7 @@ void DenseGraphAnnealer<real>::uploadJ(const real *J, const real *h, real c) {
     HIP_CHECK(hipMemcpyAsync(d_h_.ptr, h, (size_t)N_ * sizeof(real), hipMemcpyHostToDevice, dev_->stream()));
     prepareFieldForm();
     fragValid_ = false;
+    planesValid_ = false;
     c_ = c;
     h_.assign(h, h + N_);
     dev_->synchronize();
@@ -546,7 +557,7 @@ void DenseGraphAnnealer<real>::prepare() {
     HIP_CHECK(hipMemsetAsync(d_D_.ptr, 0, d_D_.bytes(), dev_->stream()));
     tag_ = 0;
     d_E_.allocate((size_t)m_);
-    d_partial_.allocate((size_t)m_ * (Np_ / 256));
+    ensureEnergyPlanes();
     d_spinDot_.allocate(1);
     d_abort_.allocate(1);
     d_counters_.allocate(16); /* [0..2] public, [3..13] optional per-phase cycles */
@@ -836,6 +847,10 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) > dev_->maxLdsPerBlock(),
                 "N=%d too large for the LDS-resident sweep (%zu bytes needed per block)", N_,
                 denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_));
+    /* multi-flip batches where LDS allows (float local-field forms but fp8) */
+    a.multiBufs = multiFlip_ ? denseSweepMultiBufs(Np_, S, sizeof(real), plainSweep_, fieldFormat_,
+                                                   dev_->maxLdsPerBlock())
+                             : 0;
     if (profile_)
         phaseBegin(2, pe);
     denseSweep<real>(a, plainSweep_, fieldFormat_, S, !coopLaunch_ && dev_->soleSweeper(), dev_->numCUs(),
@@ -921,12 +936,46 @@ AttemptStreamJob DenseGraphAnnealer<real>::takeAttemptStreamJob() {
 }
 
 template <class real>
+void DenseGraphAnnealer<real>::ensureEnergyPlanes() {
+    /* the planes and the partial sums of the energy kernels for this J and m
+     * (synchronises the stream when the planes are built) */
+    if (!planesValid_) {
+        d_planeScan_.allocate(3);
+        planes_ = energyPlaneLayout<real>(d_J_.ptr, Np_, 1, N_, N_, d_planeScan_.ptr, dev_->stream());
+        if (planes_.finite) {
+            d_planes_.allocate(energyPlaneFragCount(planes_));
+            energyPlaneBuild<real>(d_J_.ptr, Np_, 1, planes_, d_planes_.ptr, dev_->stream());
+        } else {
+            d_planes_.release();
+        }
+        planesValid_ = true;
+    }
+    size_t parts = (size_t)m_ * (Np_ / 256); /* the FP64 vector kernel's */
+    if (planes_.finite)
+        parts = std::max(parts, energyPlanePartialCount(planes_, m_));
+    d_partial_.allocate(parts);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::computeEnergies() {
+    /* exact integer planes on the MFMA units; a J holding NaN or infinity
+     * takes the FP64 vector kernel */
+    ensureEnergyPlanes();
+    if (planes_.finite)
+        energyPlanes<real>(planes_, d_planes_.ptr, d_h_.ptr, nullptr, (double)c_, d_Q_.ptr, Nw_, d_Q_.ptr, Nw_, m_,
+                           d_partial_.ptr, d_E_.ptr, dev_->stream());
+    else
+        energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
+                         dev_->stream());
+}
+
+template <class real>
 void DenseGraphAnnealer<real>::calculate_E() {
-    /* CUDADenseGraphAnnealer.cu:254-271; energies recomputed in FP64 */
+    /* CUDADenseGraphAnnealer.cu:254-271; energies evaluated exactly (integer
+     * planes) or to FP64 accuracy */
     throwErrorIfQNotSet();
     ensureDevice();
-    energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
-                     dev_->stream());
+    computeEnergies();
     std::vector<double> e((size_t)m_);
     HIP_CHECK(hipMemcpyAsync(e.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost, dev_->stream()));
     dev_->synchronize();
@@ -934,8 +983,10 @@ void DenseGraphAnnealer<real>::calculate_E() {
     const real sgn = (om_ == optMaximize) ? real(-1.) : real(1.);
     lastERoundErr_ = 0.;
     E_.resize((size_t)m_);
+    Efp64_.resize((size_t)m_);
     for (int y = 0; y < m_; ++y) {
         E_[y] = sgn * (real)e[y];
+        Efp64_[y] = (double)sgn * e[y];
         double err = std::fabs((double)(real)e[y] - e[y]);
         if (err > lastERoundErr_)
             lastERoundErr_ = err;
@@ -953,15 +1004,37 @@ double DenseGraphAnnealer<real>::formEnergyError() {
         if (fieldFormat_ == fieldF32)
             return 0.;
         std::vector<double> e((size_t)m_), ef((size_t)m_);
-        energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
-                         dev_->stream());
+        computeEnergies();
         HIP_CHECK(hipMemcpyAsync(e.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
                                  dev_->stream()));
         dev_->synchronize();
-        energyFP64<real>(d_Jsplit_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
-                         dev_->stream());
-        HIP_CHECK(hipMemcpyAsync(ef.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
-                                 dev_->stream()));
+        /* the form's J' as digit planes of its own, for this call only */
+        const EnergyPlaneLayout Lf =
+            energyPlaneLayout<float>(d_Jsplit_.ptr, Np_, 1, N_, N_, d_planeScan_.ptr, dev_->stream());
+        if (Lf.finite && planes_.finite) {
+            DevBuf<uint4> frag;
+            frag.allocate(energyPlaneFragCount(Lf));
+            energyPlaneBuild<float>(d_Jsplit_.ptr, Np_, 1, Lf, frag.ptr, dev_->stream());
+            if (energyPlanePartialCount(Lf, m_) > d_partial_.n) {
+                dev_->synchronize(); /* d_partial_ is reallocated */
+                d_partial_.allocate(energyPlanePartialCount(Lf, m_));
+            }
+            energyPlanes<float>(Lf, frag.ptr, d_h_.ptr, nullptr, (double)c_, d_Q_.ptr, Nw_, d_Q_.ptr, Nw_, m_,
+                                d_partial_.ptr, d_E_.ptr, dev_->stream());
+            HIP_CHECK(hipMemcpyAsync(ef.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
+                                     dev_->stream()));
+            dev_->synchronize(); /* before frag is freed */
+        } else {
+            energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
+                             dev_->stream());
+            HIP_CHECK(hipMemcpyAsync(e.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
+                                     dev_->stream()));
+            dev_->synchronize();
+            energyFP64<real>(d_Jsplit_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr,
+                             d_E_.ptr, dev_->stream());
+            HIP_CHECK(hipMemcpyAsync(ef.data(), d_E_.ptr, (size_t)m_ * sizeof(double), hipMemcpyDeviceToHost,
+                                     dev_->stream()));
+        }
         dev_->synchronize();
         checkAbort(true);
         double err = 0.;
@@ -1052,8 +1125,7 @@ void DenseGraphAnnealer<real>::traceSystemE(real G, real beta) {
     ++sysTraceSteps_;
     if (sysTraceLen_ >= sysTraceCap_)
         return;
-    energyFP64<real>(d_J_.ptr, d_h_.ptr, (double)c_, d_Q_.ptr, N_, Np_, m_, Nw_, d_partial_.ptr, d_E_.ptr,
-                     dev_->stream());
+    computeEnergies();
     const int sqa = isSQAAlgorithm(algo_) ? 1 : 0;
     real coef = real(0.);
     if (sqa) {

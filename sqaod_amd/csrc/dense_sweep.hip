@@ -75,6 +75,11 @@ __device__ __forceinline__ int firstLane(unsigned long long mask) {
     return mask ? (int)__builtin_ctzll(mask) : kWave;
 }
 
+/* lane l's float (l wave-uniform) */
+__device__ __forceinline__ float readLaneF(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 /* A wave-uniform int, marked so: control values the compiler cannot prove
  * uniform (they pass through loops with ballot-dependent exits) would
  * otherwise live in VGPRs and turn every branch on them into exec masking. */
@@ -248,21 +253,37 @@ __host__ __device__ constexpr size_t sweepFixedBytes() {
     return (size_t)4 * kWin * 4 + (size_t)kRing * (4 + sizeof(real)) + 16;
 }
 
-/* Bytes of LDS one wave's slice needs (the fixed part, fields, row stage,
- * pending table, spin bits and the two mirrors), for a row length Np. */
+/* Multi-flip batches (float local-field form): the J submatrix of the
+ * batch's sites, element [j][i] = J[s_i][s_j] for lanes i > j. */
+constexpr size_t kSubBytes = (size_t)kWave * kWave * 4;
+
 template <class real, int MODE>
-__host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8) {
+__host__ __device__ constexpr bool multiFlipCapable() {
+    return MODE == kModeLocal && sizeof(real) == 4;
+}
+
+/* Bytes of LDS one wave's slice needs (the fixed part, fields, row stage,
+ * pending table, spin bits and the two mirrors), for a row length Np; with
+ * multiBufs >= 2 also multiBufs - 1 more row stage buffers and the
+ * submatrix of multi-flip batches. */
+template <class real, int MODE>
+__host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8, int multiBufs = 0) {
     constexpr bool PLAIN = MODE == kModePlain;
     constexpr bool FP8 = MODE == kModeFP8;
     constexpr bool kDefer = !PLAIN && sizeof(real) == 4;
     const size_t stageBytes = kDefer ? (FP8 ? (size_t)ld8 : (size_t)Np * sizeof(real)) : 0;
     const size_t fieldBytes = PLAIN ? 0 : (size_t)Np * sizeof(real) + stageBytes;
-    return sweepFixedBytes<real>() + fieldBytes + (size_t)pendingTableSize(Np) * 4 + (size_t)3 * (Np / 32) * 4;
+    size_t b = sweepFixedBytes<real>() + fieldBytes + (size_t)pendingTableSize(Np) * 4 + (size_t)3 * (Np / 32) * 4;
+    if (multiFlipCapable<real, MODE>() && multiBufs >= 2)
+        b += (size_t)(multiBufs - 1) * stageBytes + kSubBytes;
+    return b;
 }
 
 /* One wave runs the whole sweep of slice y: N attempts, with its LDS region
- * p (sweepWaveBytes), the exp table and h already in LDS. */
-template <class real, int MODE>
+ * p (sweepWaveBytes), the exp table and h already in LDS.  MULTI compiles the
+ * multi-flip batches in (float local-field form only); without it the batch
+ * ends at its first accepted flip. */
+template <class real, int MODE, bool MULTI = false>
 __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y, const uint64_t *expTab,
                                            const real *hS, unsigned char *p) {
 #pragma clang fp contract(off)
@@ -294,6 +315,30 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     const uint32_t stageL = winDL + (uint32_t)(sweepFixedBytes<real>() + (size_t)Np * sizeof(real));
     (void)scaleL;
     (void)stageL;
+    /* Multi-flip batches: buffers 1 .. nBufs-1 of row stage after the
+     * mirrors, then the submatrix.  A batch in multi mode gathers, for its 64
+     * sites, sub[j][i] = J[s_i][s_j] (i > j) by DMA while the neighbour phase
+     * runs; its lanes then decide in order, each accepted flip correcting the
+     * fields of the later lanes with that flip's column of the submatrix,
+     * and the commit applies the batch's flipped rows to F through the stage
+     * buffers (the last one deferred like a single flip's). */
+    constexpr bool kMultiCap = MULTI && multiFlipCapable<real, MODE>();
+    const int nBufs = kMultiCap ? uni(a.multiBufs) : 1;
+    unsigned char *const multiBase = reinterpret_cast<unsigned char *>(Mir[1] + Nw);
+    real *const sub = reinterpret_cast<real *>(multiBase + (size_t)(nBufs - 1) * stageBytes);
+    const uint32_t multiBaseL = ldsAddress(multiBase);
+    const uint32_t subL = multiBaseL + (uint32_t)((size_t)(nBufs - 1) * stageBytes);
+    (void)sub;
+    (void)subL;
+    /* row stage buffer b: buffer 0 is `stage`, the others follow the mirrors */
+    auto stageBuf = [&](int b) -> real * {
+        return b == 0 ? stage : reinterpret_cast<real *>(multiBase + (size_t)(b - 1) * stageBytes);
+    };
+    auto stageBufL = [&](int b) -> uint32_t {
+        return b == 0 ? stageL : multiBaseL + (uint32_t)((size_t)(b - 1) * stageBytes);
+    };
+    (void)stageBuf;
+    (void)stageBufL;
 
     const long long yN = (long long)y * N;
     const int n0 = (y == 0) ? m - 1 : y - 1;
@@ -335,11 +380,16 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
      * float the fp32 form adds for the same J'. */
     bool rowPending = false;
     real rowD = 0;
-    /* four 256-element slices per round: all their LDS reads are issued
-     * before the first write, so the round trips overlap (Np is a multiple
-     * of 256; the tail round covers what is left) */
-    auto applyPendingRow = [&]() {
+    int curStage = 0; /* the stage buffer holding the pending row */
+    /* F += d * (row in stage buffer b), four 256-element slices per round:
+     * all their LDS reads are issued before the first write, so the round
+     * trips overlap (Np is a multiple of 256; the tail round covers what is
+     * left) */
+    auto applyRow = [&](int b, real d) {
         constexpr int kR = 4;
+        const real *sb = stageBuf(b);
+        const uint8_t *sb8 = reinterpret_cast<const uint8_t *>(sb);
+        (void)sb8;
         /* b0 and the slice bounds are wave-uniform: scalar loop and branches */
         for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
             real fv[kR][4], sv[kR][4];
@@ -349,9 +399,9 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 if (b0 + r * kWave * 4 < Np) {
                     ldsLoad4(F + j, fv[r]);
                     if constexpr (FP8)
-                        fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(stage8 + j), sv[r]);
+                        fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(sb8 + j), sv[r]);
                     else
-                        ldsLoad4(stage + j, sv[r]);
+                        ldsLoad4(sb + j, sv[r]);
                 }
             }
 #pragma unroll
@@ -360,11 +410,14 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 if (b0 + r * kWave * 4 < Np) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        fv[r][t] = fv[r][t] + rowD * sv[r][t];
+                        fv[r][t] = fv[r][t] + d * sv[r][t];
                     ldsStore4(F + j, fv[r]);
                 }
             }
         }
+    };
+    auto applyPendingRow = [&]() {
+        applyRow(curStage, rowD);
         rowPending = false;
     };
 
@@ -432,6 +485,9 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
      * the pending row's application, the next row's DMA issue */
     unsigned long long cyc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     (void)cyc;
+    /* multi mode for the next batch: chosen from how many flips the last
+     * batch committed (several, or one within its first lanes) */
+    bool multi = false;
     while (k < N) {
         SQ_CYC(c0);
         SQ_CYC_WAITALL();
@@ -443,6 +499,20 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         const real u = ringU[(k + lane) & (kRing - 1)];
         if (k + lane >= N)
             s = 0u;
+        if constexpr (kMultiCap) {
+            if (multi) {
+                /* sub[j][i] = J[s_i][s_j] for lanes i > j: 63 DMA ops of 64
+                 * scattered dwords, landing while the neighbour phase runs
+                 * (waited for with the staged row below) */
+                ldsWait(); /* this wave's reads of sub (last batch) are done */
+                const real *myRow = a.J + (long long)s * Np;
+                for (int j = 0; j < kWave - 1; ++j) {
+                    const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)s, j);
+                    if (lane > j)
+                        dmaDword<false>(myRow + sj, subL + (uint32_t)j * (kWave * 4));
+                }
+            }
+        }
         uint32_t nd[2][2]; /* the windows, read together */
         auto readWindow = [&](int nb) {
             nd[nb][0] = winD[nb * kWin + lane];
@@ -542,15 +612,17 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
          * it before the next windows are requested (vmcnt counts in order) */
         real rowCorr = 0;
         if constexpr (kDefer) {
-            if (rowPending) {
+            if (rowPending || multi) {
                 vmWaitImm<0>();
-                if constexpr (FP8) {
-                    rowD = rowSign * scaleSlot[0];
-                    float v4[4];
-                    fp8x4ToFloat(stage8[s], v4);
-                    rowCorr = v4[0];
-                } else {
-                    rowCorr = stage[s];
+                if (rowPending) {
+                    if constexpr (FP8) {
+                        rowD = rowSign * scaleSlot[0];
+                        float v4[4];
+                        fp8x4ToFloat(stage8[s], v4);
+                        rowCorr = v4[0];
+                    } else {
+                        rowCorr = stageBuf(curStage)[s];
+                    }
                 }
             }
         }
@@ -579,9 +651,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                     hypNb = code & 1;
             }
         }
+        const bool blockedMulti = blocked; /* a slot of several pending entries: waits in any case */
         const uint32_t qword = Qb[s >> 5];
         const bool qpos = (qword >> (s & 31)) & 1u;
-        const real q = qpos ? (real)1 : (real)-1;
+        real q = qpos ? (real)1 : (real)-1; /* multi mode: negated after an earlier flip at this site */
         const real hx = hS[s];
         real nsum = 0, nsumAlt = 0;
         if (!a.sa) {
@@ -610,6 +683,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         };
         auto decide = [&](real f) { return decideWith(f, nsum); };
         bool accLocal = false;
+        real fLane = 0; /* this lane's field as its decision used it */
         if constexpr (PLAIN) {
             blocked = blocked || hypNb >= 0;
         } else {
@@ -622,10 +696,16 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             accLocal = decide(f);
             if (hypNb >= 0 && decideWith(f, nsumAlt) != accLocal)
                 blocked = true;
+            fLane = f;
         }
+        (void)fLane;
+        (void)blockedMulti;
         const unsigned long long stopMask = __ballot(!valid || blocked);
         const int firstStop = uni(firstLane(stopMask));
         int c = firstStop, flipLane = -1;
+        unsigned long long flipMask = 0; /* the committed flips, by lane */
+        real dFlip = 0;                  /* a flipped lane's row factor -2 q (q before its flip) */
+        int nCand = 0;                   /* lanes that would accept (the next batch's mode) */
         if constexpr (PLAIN) {
             /* reference algorithm: field = J[s] . q_y recomputed per attempt,
              * as coalesced wave-wide dots, eight attempts at a time, stopping
@@ -667,15 +747,57 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 if (accMask)
                     flipLane = uni(firstLane(accMask));
             }
-            if (flipLane >= 0)
+            if (flipLane >= 0) {
                 c = flipLane + 1;
+                flipMask = 1ull << flipLane;
+            }
+        } else if (kMultiCap && multi) {
+            /* lanes decide in order; after each accepted flip (lane fa, site
+             * x, factor d = -2 q) the later lanes' fields get the serial
+             * update F[s] = F[s] + d J[x][s] (J[x][s] = sub[fa][lane]), a
+             * later attempt at site x sees the flipped spin, and their
+             * decisions and two-outcome checks are redone.  Lanes before the
+             * next accepted one are confirmed rejections; the batch ends at
+             * the first lane that is blocked or past the window. */
+            int j = 0;
+            c = kWave;
+            for (;;) {
+                const bool act = lane >= j;
+                const int fs = uni(firstLane(__ballot(act && (!valid || blocked))));
+                const int fa = uni(firstLane(__ballot(act && valid && !blocked && accLocal)));
+                if (fa >= fs) {
+                    c = fs;
+                    break;
+                }
+                flipMask |= 1ull << fa;
+                const uint32_t xa = (uint32_t)__builtin_amdgcn_readlane((int)s, fa);
+                const real da = __builtin_amdgcn_readlane((int)(q > (real)0), fa) ? (real)-2. : (real)2.;
+                if (lane == fa)
+                    dFlip = da;
+                if (lane > fa) {
+                    fLane = fLane + da * sub[fa * kWave + lane];
+                    if (s == xa)
+                        q = -q;
+                    accLocal = decide(fLane);
+                    blocked = blockedMulti || (hypNb >= 0 && decideWith(fLane, nsumAlt) != accLocal);
+                }
+                j = fa + 1;
+                if (j >= kWave)
+                    break;
+            }
+            flipLane = flipMask ? uni(63 - __builtin_clzll(flipMask)) : -1; /* the last flip */
+            nCand = __popcll(flipMask);
         } else {
             const bool acc = accLocal;
             const unsigned long long accMask = __ballot(valid && !blocked && acc);
             const int firstAcc = uni(firstLane(accMask));
+            nCand = __popcll(accMask);
             if (firstAcc < firstStop) {
                 c = firstAcc + 1;
                 flipLane = firstAcc;
+                flipMask = 1ull << firstAcc;
+                if (lane == firstAcc)
+                    dFlip = qpos ? (real)-2. : (real)2.;
             }
         }
         waveSync();
@@ -750,7 +872,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
 
         /* publish the committed decisions */
         if (lane < c) /* lane 0 always stores: one op */
-            storeShared32(a.D + yN + k + lane, (tag << 1) | (lane == flipLane ? 1u : 0u));
+            storeShared32(a.D + yN + k + lane, (tag << 1) | (uint32_t)((flipMask >> lane) & 1ull));
         afterGroup += 1;
 
         SQ_CYC(c4);
@@ -762,9 +884,14 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         if (flipLane >= 0) {
             const uint32_t x = __shfl(s, flipLane);
             const bool qx = __shfl((int)qpos, flipLane) != 0;
-            ++nFlips;
-            if (lane == 0)
-                atomicXor(&Qb[x >> 5], 1u << (x & 31));
+            const int nFl = kMultiCap ? uni(__popcll(flipMask)) : 1;
+            nFlips += nFl;
+            /* every flipped lane toggles its own site's bit (a site flipped
+             * twice in one batch toggles twice) */
+            if ((flipMask >> lane) & 1ull)
+                atomicXor(&Qb[s >> 5], 1u << (s & 31));
+            (void)x;
+            (void)qx;
             if constexpr (FP8) {
                 /* the same deferred update from the e4m3 row (ld8 bytes, a
                  * multiple of 1 KiB) and its scale, rowD = -+2 * scale */
@@ -779,15 +906,49 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 afterGroup += rowOps;
                 rowPending = true;
             } else if constexpr (kDefer) {
-                /* F[y][:] += -2 q_x J[x][:] (J symmetric), deferred: the row
-                 * streams into `stage`, 1 KiB (16 B per lane) per op */
-                rowD = qx ? (real)-2. : (real)2.;
-                const real *row = a.J + (long long)x * Np;
+                /* F[y][:] += -2 q_x J[x][:] (J symmetric) for the flips in lane
+                 * order.  Rows stream into the stage buffers (1 KiB, 16 B per
+                 * lane, per op), nBufs in flight; every row but the last is
+                 * applied to F here, the last is deferred: the next batch's
+                 * lanes add rowD * stage[s] to F[s] themselves and the next
+                 * commit applies it.  Row r is in buffer r % nBufs; a buffer is
+                 * refilled only after its row was applied, and vmcnt retires in
+                 * issue order, so row r has landed once at most (issued - 1 -
+                 * r) rows' ops are outstanding ((nBufs - 1) rows' ops fit the
+                 * counter). */
                 waveSync();
-                ldsWait(); /* this wave's reads of `stage` are done */
-                for (int j0 = 0; j0 < Np; j0 += kWave * 4)
-                    dmaDwordx4(row + j0 + lane * 4, stageL + j0 * 4);
-                afterGroup += rowOps;
+                ldsWait(); /* this wave's reads of the stage buffers are done */
+                unsigned long long issueMask = flipMask, applyMask = flipMask;
+                /* buffer indices wrap by compare (r % nBufs without a divide) */
+                int issued = 0, applied = 0, issueBuf = 0, applyBuf = 0;
+                auto issueNext = [&]() {
+                    const int fl = uni(firstLane(issueMask));
+                    issueMask &= issueMask - 1;
+                    const uint32_t xr = (uint32_t)__builtin_amdgcn_readlane((int)s, fl);
+                    const real *row = a.J + (long long)xr * Np;
+                    const uint32_t dst = stageBufL(issueBuf);
+                    for (int j0 = 0; j0 < Np; j0 += kWave * 4)
+                        dmaDwordx4(row + j0 + lane * 4, dst + j0 * 4);
+                    afterGroup += rowOps;
+                    ++issued;
+                    issueBuf = (issueBuf + 1 == nBufs) ? 0 : issueBuf + 1;
+                };
+                while (issued < nFl && issued < nBufs)
+                    issueNext();
+                while (applied < nFl - 1) {
+                    vmWait((issued - 1 - applied) * rowOps);
+                    const int fl = uni(firstLane(applyMask));
+                    applyMask &= applyMask - 1;
+                    applyRow(applyBuf, readLaneF(dFlip, fl));
+                    ++applied;
+                    applyBuf = (applyBuf + 1 == nBufs) ? 0 : applyBuf + 1;
+                    if (issued < nFl) {
+                        ldsWait(); /* the applied buffer's reads are done */
+                        issueNext();
+                    }
+                }
+                rowD = readLaneF(dFlip, uni(firstLane(applyMask)));
+                curStage = applyBuf; /* the last row's buffer */
                 rowPending = true;
             } else if constexpr (!PLAIN) {
                 /* F[y][:] += -2 q_x J[:][x] = -2 q_x J[x][:]  (J symmetric);
@@ -820,6 +981,12 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         }
         SQ_CYC(r2);
         SQ_CYC_ADD(cyc[10], r1, r2);
+        if constexpr (kMultiCap) {
+            /* three or more lanes accepting (flips of a multi batch, or
+             * candidates of a single-flip batch): the next batch resolves
+             * its flips in order instead of ending at the first */
+            multi = nBufs >= 2 && nCand >= 3;
+        }
         k = uni(k + c);
         waveSync();
         SQ_CYC(c5);
@@ -861,7 +1028,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     }
 }
 
-template <class real, int MODE>
+template <class real, int MODE, bool MULTI = false>
 __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
                                                                                 int xcdRemap) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -879,7 +1046,8 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     /* LDS carve-up: exp table, h, then one region per wave */
     uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
     real *hS = reinterpret_cast<real *>(lds + kExpTabBytes);
-    unsigned char *p = lds + kExpTabBytes + (size_t)a.Np * sizeof(real) + sweepWaveBytes<real, MODE>(a.Np, a.ld8) * w;
+    const size_t waveBytes = sweepWaveBytes<real, MODE>(a.Np, a.ld8, MULTI ? a.multiBufs : 0);
+    unsigned char *p = lds + kExpTabBytes + (size_t)a.Np * sizeof(real) + waveBytes * w;
     if (threadIdx.x < 32)
         expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
     {
@@ -895,7 +1063,7 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     __syncthreads();
     if (y >= a.m)
         return;
-    sweepSlice<real, MODE>(a, y, expTab, hS, p);
+    sweepSlice<real, MODE, MULTI>(a, y, expTab, hS, p);
 }
 
 /* Many instances in one persistent launch: every wave takes (instance,
@@ -969,12 +1137,13 @@ void fp8DecodeTable(float out[256]) {
 }
 
 namespace {
-size_t waveBytesFor(int Np, size_t realSize, bool plain, int fieldFormat, int ld8) {
+size_t waveBytesFor(int Np, size_t realSize, bool plain, int fieldFormat, int ld8, int multiBufs = 0) {
     if (realSize == 8)
         return plain ? sweepWaveBytes<double, kModePlain>(Np, ld8) : sweepWaveBytes<double, kModeLocal>(Np, ld8);
     if (plain)
         return sweepWaveBytes<float, kModePlain>(Np, ld8);
-    return fieldFormat == fieldFP8 ? sweepWaveBytes<float, kModeFP8>(Np, ld8) : sweepWaveBytes<float, kModeLocal>(Np, ld8);
+    return fieldFormat == fieldFP8 ? sweepWaveBytes<float, kModeFP8>(Np, ld8)
+                                   : sweepWaveBytes<float, kModeLocal>(Np, ld8, multiBufs);
 }
 
 template <class real>
@@ -988,8 +1157,21 @@ void *batchKernelFor(bool plain, int fieldFormat) {
 }
 }  // namespace
 
-size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, int ld8) {
-    return kExpTabBytes + (size_t)Np * realSize + waveBytesFor(Np, realSize, plain, fieldFormat, ld8) * slicesPerBlock;
+size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, int ld8,
+                          int multiBufs) {
+    return kExpTabBytes + (size_t)Np * realSize +
+           waveBytesFor(Np, realSize, plain, fieldFormat, ld8, multiBufs) * slicesPerBlock;
+}
+
+int denseSweepMultiBufs(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, size_t maxLds) {
+    if (realSize != 4 || plain || fieldFormat == fieldFP8)
+        return 0;
+    const int rowOps = Np * 4 / (kWave * 16); /* 1 KiB DMA ops per row */
+    for (int b = 4; b >= 2; --b)
+        if ((b - 1) * rowOps <= 63 &&
+            denseSweepLdsBytes(Np, slicesPerBlock, realSize, plain, fieldFormat, 0, b) <= maxLds)
+            return b;
+    return 0;
 }
 
 size_t denseSweepBatchRegionBytes(int maxNp, int maxLd8, size_t realSize, bool plain, int fieldFormat) {
@@ -1072,13 +1254,17 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
     SQ_THROW_IF(fp8 && (sizeof(real) != 4 || !args.J8 || !args.rowScale || args.ld8 < args.Np || args.ld8 % 1024),
                 "fp8 sweep: needs a float solver and e4m3 rows with a 1 KiB-multiple stride");
     int nBlocks = divru(args.m, slicesPerBlock);
-    size_t lds = denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain, fieldFormat, args.ld8);
+    size_t lds =
+        denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain, fieldFormat, args.ld8, args.multiBufs);
     int xcdRemap = (nBlocks % 8 == 0 && nBlocks >= 16) ? 1 : 0;
 
     void *fn = plain ? (void *)denseSweepKernel<real, kModePlain> : (void *)denseSweepKernel<real, kModeLocal>;
-    if constexpr (sizeof(real) == 4)
+    if constexpr (sizeof(real) == 4) {
         if (fp8)
             fn = (void *)denseSweepKernel<real, kModeFP8>;
+        else if (!plain && args.multiBufs >= 2)
+            fn = (void *)denseSweepKernel<real, kModeLocal, true>;
+    }
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SweepArgs<real> a = args;
     void *kargs[] = {(void *)&a, (void *)&slicesPerBlock, (void *)&xcdRemap};

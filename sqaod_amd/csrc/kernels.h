@@ -262,6 +262,8 @@ struct SweepArgs {
     int *abortHost;         /* host-visible copy of abortFlag (pinned host memory), or null */
     unsigned long long *counters; /* [0] accepted flips, [1] batches, [2] blocked waits */
     int spinLimit;
+    int multiBufs;          /* float local-field form: row stage buffers for batches that commit several
+                             * flips (>= 2), 0: every batch ends at its first flip */
 };
 
 /* One annealOneStep sweep: N attempts per slice with exact serial-coloring
@@ -276,7 +278,11 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
 /* Bytes of LDS one block of the local-field sweep needs (ld8: the fp8 row
  * stride, used when fieldFormat == fieldFP8). */
 size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat = fieldF32,
-                          int ld8 = 0);
+                          int ld8 = 0, int multiBufs = 0);
+/* Row stage buffers for multi-flip batches that fit beside S slices' regions
+ * in maxLds bytes (0: none; at most 4, and (buffers - 1) rows of DMA ops within
+ * the 63 outstanding memory operations a wave can count). */
+int denseSweepMultiBufs(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, size_t maxLds);
 
 /* Many instances in one persistent launch (one wave per (instance, slice)
  * item, taken from a queue in item order; items of one instance must be
@@ -337,6 +343,35 @@ template <class real>
 void energyBipartiteFP64(const real *d_JT, const real *d_h0, const real *d_h1, double c, const uint32_t *d_Q0,
                          const uint32_t *d_Q1, int N0, int N1, int N0p, int N1p, int m, double *d_partial,
                          double *d_E, hipStream_t stream);
+
+/* Energies on the integer MFMA units (energy_planes.hip).  B (nOut x K,
+ * element (i, k) at B[i sRow + k sCol]) is held as P int8 digit planes:
+ * B = 2^es sum_p 256^p D_p, exact when `exact` (every element a multiple of
+ * 2^es within 8 P - 2 bits), else rounded at 2^es = 2^(max exponent + 3 -
+ * 8 P).  finite = 0 if B holds a NaN or an infinity (the planes are then not
+ * usable). */
+struct EnergyPlaneLayout {
+    int nOut = 0, K = 0;
+    int outTiles = 0, kBlocks = 0; /* 16-row tiles, 64-column blocks */
+    int P = 0, es = 0, exact = 0, finite = 1;
+};
+/* Scans B and picks the layout; d_scan holds 3 ints; synchronises `stream`. */
+template <class real>
+EnergyPlaneLayout energyPlaneLayout(const real *d_B, long long sRow, long long sCol, int nOut, int K, int *d_scan,
+                                    hipStream_t stream);
+/* uint4 elements of the planes; doubles of the partial sums for m slices */
+size_t energyPlaneFragCount(const EnergyPlaneLayout &L);
+size_t energyPlanePartialCount(const EnergyPlaneLayout &L, int m);
+template <class real>
+void energyPlaneBuild(const real *d_B, long long sRow, long long sCol, const EnergyPlaneLayout &L, void *d_frag,
+                      hipStream_t stream);
+/* E[y] = -c - sum_i qout_yi (hOut_i + sum_k B_ik qin_yk) - hIn . qin_y (hIn
+ * may be null), spins from packed bits (NwIn, NwOut words per slice row, at
+ * least 2 kBlocks and ceil(outTiles / 2) of them). */
+template <class real>
+void energyPlanes(const EnergyPlaneLayout &L, const void *d_frag, const real *d_hOut, const real *d_hIn, double c,
+                  const uint32_t *d_Qin, int NwIn, const uint32_t *d_Qout, int NwOut, int m, double *d_partial,
+                  double *d_E, hipStream_t stream);
 
 /* Batched bilinear forms for the formulas objects (energy.hip):
  * E[o] = cst + sgn (lin0 . v + sum_i u_i (lin1_i + sum_j A[i][j] v_j)) with u, v

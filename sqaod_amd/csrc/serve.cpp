@@ -130,6 +130,7 @@ size_t Server::estimateDenseBytes(int N, int m, int fieldFormat) {
         b += Np * Np * 4;                                           /* fp32 fragments */
     b += (size_t)m * Np * 4 + 3 * (size_t)m * n * 4 + (size_t)m * Np / 8; /* F, X, U, D, Q */
     b += (size_t)m * (Np / 256 + 2) * 8 + (size_t)m * n;            /* energies, q for makeSolution */
+    b += 6 * Np * Np + (size_t)m * (2048 + Np / 16) * 8;            /* energy planes (at most 6), partials */
     return b + b / 16 + ((size_t)1 << 20);
 }
 
@@ -141,6 +142,7 @@ size_t Server::estimateBipartiteBytes(int N0, int N1, int m, int fieldFormat) {
     b += (size_t)m * std::max(N0p, N1p) * 4;              /* dEmat */
     b += (size_t)m * (N0p + N1p) / 8;                     /* spins */
     b += (size_t)m * ((N0p + N1p) / 256 + 2) * 8 + (size_t)m * (N0 + N1);
+    b += 6 * N0p * N1p + (size_t)m * (2048 + N1p / 16) * 8;         /* energy planes (at most 6), partials */
     return b + b / 16 + ((size_t)1 << 20);
 }
 

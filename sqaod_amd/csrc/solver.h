@@ -151,6 +151,8 @@ public:
     /* Max |E_real - E_fp64| of the last calculate_E (the rounding of the
      * FP64 recompute to real). */
     double lastEnergyRoundingError() const { return lastERoundErr_; }
+    /* The energies of the last calculate_E before their rounding to real. */
+    const std::vector<double> &energiesFP64() const { return Efp64_; }
     /* Max over slices of |E_form(q_y) - E(q_y)| for the current spins, both
      * in FP64: E_form uses the J the field form anneals (bf16 hi + lo, or the
      * fp8 J'), E the J as given; 0 for the fp32/fp64 form. */
@@ -225,6 +227,7 @@ private:
     bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
     int spinLimit_ = 1 << 20;
     int nWorkers_ = 1;        /* CPU workers whose MT19937 streams the coloring step replays */
+    bool multiFlip_ = false;  /* multi-flip batches in the float local-field sweep */
     int sysTraceCap_ = 0, sysTraceLen_ = 0;
     long long sysTraceSteps_ = 0;
     int mtWorkersSeeded_ = 0; /* generators in d_mt_ */
@@ -247,6 +250,7 @@ private:
     bool fragValid_ = false; /* d_frag16_/d_frag32_ hold the current J in MFMA fragment order */
     int stepsSinceRefresh_ = 0;
     double lastERoundErr_ = 0.;
+    std::vector<double> Efp64_;
 
     /* device state */
     DevBuf<real> d_J_, d_h_, d_F_, d_U_;
@@ -261,6 +265,13 @@ private:
     DevBuf<uint32_t> d_Q_, d_X_, d_D_, d_words_;
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
+    /* energies: J as int8 digit planes for the integer MFMA, built once per J */
+    EnergyPlaneLayout planes_;
+    DevBuf<uint4> d_planes_;
+    DevBuf<int> d_planeScan_;
+    bool planesValid_ = false;
+    void ensureEnergyPlanes();
+    void computeEnergies(); /* E of every slice into d_E_, on the stream */
     DevBuf<real> d_sysTrace_;
     DevBuf<int> d_abort_;
     DevBuf<unsigned long long> d_counters_;
@@ -485,6 +496,8 @@ public:
     void randomizeSpin();
     void prepare();
     void calculate_E();
+    /* The energies of the last calculate_E before their rounding to real. */
+    const std::vector<double> &energiesFP64() const { return Efp64_; }
     void makeSolution();
     real getSystemE(real G, real beta);
     void annealOneStep(real G, real beta);
@@ -515,6 +528,7 @@ private:
 
     std::vector<real> h0_, h1_;
     std::vector<real> E_;
+    std::vector<double> Efp64_;
     std::vector<int8_t> q0_, q1_;
 
     DevBuf<real> d_J_, d_JT_, d_h0_, d_h1_, d_dE_;
@@ -524,6 +538,13 @@ private:
     DevBuf<MTState> d_mt_;
     DevBuf<double> d_E_, d_partial_, d_spinDot_;
     DevBuf<int8_t> d_q8_;
+    /* energies: J (rows for q1) as int8 digit planes for the integer MFMA */
+    EnergyPlaneLayout planes_;
+    DevBuf<uint4> d_planes_;
+    DevBuf<int> d_planeScan_;
+    bool planesValid_ = false;
+    void ensureEnergyPlanes();
+    void computeEnergies();
 };
 
 /* ------------------------------------------------------------------ */

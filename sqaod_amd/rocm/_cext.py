@@ -54,6 +54,7 @@ _SIGS = {
     "sqaod_rocm_dg_annealer_anneal_one_step": (_i32, [_vp, _f64, _f64]),
     "sqaod_rocm_dg_annealer_get_counters": (_i32, [_vp, ctypes.POINTER(ctypes.c_ulonglong)]),
     "sqaod_rocm_dg_annealer_get_energy_rounding_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "sqaod_rocm_dg_annealer_get_E_fp64": (_i32, [_vp, _vp, _i32]),
     "sqaod_rocm_dg_annealer_get_form_energy_error": (_i32, [_vp, ctypes.POINTER(_f64)]),
     "sqaod_rocm_dg_annealer_set_system_E_trace": (_i32, [_vp, _i32]),
     "sqaod_rocm_dg_annealer_get_system_E_trace": (_i32, [_vp, _vp, _i32, ctypes.POINTER(_i32),
@@ -87,6 +88,7 @@ _SIGS = {
     "sqaod_rocm_bg_annealer_get_preferences": (_i32, [_vp, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
     "sqaod_rocm_bg_annealer_get_n_trotters": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "sqaod_rocm_bg_annealer_get_E": (_i32, [_vp, _vp, _i32]),
+    "sqaod_rocm_bg_annealer_get_E_fp64": (_i32, [_vp, _vp, _i32]),
     "sqaod_rocm_bg_annealer_get_x": (_i32, [_vp, _vp, _vp, _i32, _i32, _i32]),
     "sqaod_rocm_bg_annealer_get_hamiltonian": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "sqaod_rocm_bg_annealer_get_q": (_i32, [_vp, _vp, _vp, _i32, _i32, _i32]),
@@ -408,6 +410,14 @@ class _DenseGraphAnnealer:
         return out[:n.value], steps.value
 
     @staticmethod
+    def get_E_fp64(cobj, dtype):
+        m = _i32()
+        _check(_lib.sqaod_rocm_dg_annealer_get_n_trotters(cobj, ctypes.byref(m)))
+        E = np.empty(m.value, np.float64)
+        _check(_lib.sqaod_rocm_dg_annealer_get_E_fp64(cobj, _ptr(E), m.value))
+        return E
+
+    @staticmethod
     def get_energy_rounding_error(cobj, dtype):
         e = _f64()
         _check(_lib.sqaod_rocm_dg_annealer_get_energy_rounding_error(cobj, ctypes.byref(e)))
@@ -648,6 +658,13 @@ class _BipartiteGraphAnnealer:
         m = _BipartiteGraphAnnealer._m(cobj)
         E = np.empty(m, dtype)
         _check(_lib.sqaod_rocm_bg_annealer_get_E(cobj, _ptr(E), m))
+        return E
+
+    @staticmethod
+    def get_E_fp64(cobj, dtype):
+        m = _BipartiteGraphAnnealer._m(cobj)
+        E = np.empty(m, np.float64)
+        _check(_lib.sqaod_rocm_bg_annealer_get_E_fp64(cobj, _ptr(E), m))
         return E
 
     @staticmethod

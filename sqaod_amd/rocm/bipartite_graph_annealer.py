@@ -17,6 +17,11 @@ class BipartiteGraphAnnealer(BipartiteGraphAnnealerBase):
         self._device = dev
         BipartiteGraphAnnealerBase.__init__(self, cext, dtype, (b0, b1, W), optimize, prefdict)
 
+    def get_E_fp64(self):
+        """E of every slice from the last energy evaluation, in float64 before
+        its rounding to dtype (exact on integer problems)."""
+        return cext.get_E_fp64(self._cobj, self.dtype)
+
 
 def bipartite_graph_annealer(b0=None, b1=None, W=None, optimize=minimize, dtype=np.float64, **prefs):
     """Factory of sqaod_amd.rocm.BipartiteGraphAnnealer (same signature as

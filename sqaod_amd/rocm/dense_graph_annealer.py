@@ -41,6 +41,11 @@ class DenseGraphAnnealer(DenseGraphAnnealerBase):
         """The recorded values (float64 array) and the number of steps traced."""
         return cext.get_system_E_trace(self._cobj, getattr(self, "_trace_cap", 0), self.dtype)
 
+    def get_E_fp64(self):
+        """E of every slice from the last energy evaluation, in float64 before
+        its rounding to dtype (exact on integer problems)."""
+        return cext.get_E_fp64(self._cobj, self.dtype)
+
     def get_energy_rounding_error(self):
         """max |E(dtype) - E(fp64)| of the last energy evaluation."""
         return cext.get_energy_rounding_error(self._cobj, self.dtype)

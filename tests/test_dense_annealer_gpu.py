@@ -308,3 +308,31 @@ def test_generators_continue_across_prepare(oracle, rocm):
             ora.anneal_one_step(G, BETA)
             an.anneal_one_step(G, BETA)
         assert np.array_equal(q_of(an), ora.get_q()), rep
+
+
+@pytest.mark.parametrize("N,m,form,algo", [(128, 32, "fp32", "coloring"), (128, 32, "bf16x2", "coloring"),
+                                           (300, 7, "fp32", "coloring"), (64, 2, "fp32", "coloring"),
+                                           (1000, 16, "bf16x2", "coloring"), (96, 1, "fp32", "sa_naive"),
+                                           (2048, 64, "fp32", "coloring")])
+def test_multi_flip_batches_bit_exact(oracle, rocm, N, m, form, algo):
+    """multi_flip=on: a batch resolves all its accepted flips in order (each
+    flip corrects the later lanes' fields with the gathered submatrix and the
+    rows are applied in flip order), so the trajectory is still the serial
+    CPU annealer's, bit for bit, in the flip-heavy start of the schedule and
+    to its end."""
+    W = dense_int_W(N, SEED_BASE + 3 * N + m)
+    ora, an = make_pair(oracle, rocm, W, m, seed=31 + m, algo=algo, field_precision=form, multi_flip="on")
+    assert an.get_preferences()["multi_flip"] == "on"
+    sch = schedule() if N <= 300 else schedule()[:40]
+    if algo == "sa_naive":
+        sch = list(np.geomspace(20.0, 0.05, 60))
+    for k, G in enumerate(sch):
+        beta = 0.0 if algo == "sa_naive" else BETA
+        ora.anneal_one_step(G, beta)
+        an.anneal_one_step(G, beta)
+        if k % 10 == 0 or k == len(sch) - 1:
+            qa, qo = q_of(an), ora.get_q()
+            assert np.array_equal(qa, qo), f"step {k}: {int(np.sum(qa != qo))} spins differ"
+    assert np.array_equal(an.get_E(), ora.calculate_E())
+    c = an.get_counters()
+    assert c["flips"] > 0

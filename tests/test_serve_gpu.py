@@ -94,18 +94,18 @@ def test_results_equal_jobs_run_alone(rocm):
 
 def test_memory_budget_admission_in_arrival_order(rocm):
     """A budget that holds only some of the jobs at a time (dense N = 64, 300,
-    1000 take about 19 MiB by the admission estimate, N = 2048 about 53 MiB,
-    the budget is 60 MiB): the others wait, the estimate in use never exceeds
+    1000 take about 28 MiB by the admission estimate, N = 2048 about 78 MiB,
+    the budget is 90 MiB): the others wait, the estimate in use never exceeds
     the budget, admission follows arrival order (the last small job waits
     behind the large one), and every job completes."""
     js = jobs()[:4] + jobs()[:1]
-    srv = rocm.server(memory_budget_mb=60, max_active=3)
+    srv = rocm.server(memory_budget_mb=90, max_active=3)
     ids = [submit(srv, j) for j in js]
     peak_active = 0
     while True:
         rem = srv.tick()
         st = srv.status()
-        assert st["bytes_in_use"] <= 60 << 20
+        assert st["bytes_in_use"] <= 90 << 20
         peak_active = max(peak_active, st["active"])
         if rem == 0:
             break
@@ -114,7 +114,7 @@ def test_memory_budget_admission_in_arrival_order(rocm):
     assert peak_active <= 3
     admits = [srv.result(i)["admit_s"] for i in ids]
     assert admits == sorted(admits)
-    assert srv.status()["peak_bytes"] <= 60 << 20
+    assert srv.status()["peak_bytes"] <= 90 << 20
     r = [srv.result(i) for i in ids]
     # N = 2048 fits only once N = 1000 (the first job to finish) has left
     assert r[3]["admit_s"] >= r[2]["finish_s"]
