@@ -406,8 +406,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--multi-flip", choices=("on", "off"), default=None,
-                    help="multi-flip batches of the float local-field forms (default: the library's)")
+    ap.add_argument("--multi-flip", default=None,
+                    help="multi-flip batches of the float local-field forms: auto, on, off or the "
+                         "threshold 2..64 (default: the library's, auto)")
     ap.add_argument("--no-extras", action="store_true", help="skip the bipartite, BF search and serving runs")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--batch-instances", type=int, default=512)

@@ -82,9 +82,12 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
  * "n_workers" (int >= 1: with rng = mt19937, the per-worker streams of the
  * CPU annealer's OpenMP coloring step with that many threads),
  * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2" | "fp8"),
- * "multi_flip" ("off" | "on": float local-field sweeps commit every flip of a
- * 64-attempt batch in order instead of ending the batch at its first flip;
- * same results, faster only while many attempts are accepted),
+ * "multi_flip" ("auto" | "on" | "off" | a number 2..64: float local-field
+ * sweeps commit every flip of a 64-attempt batch in order, instead of ending
+ * the batch at its first flip, after a batch with at least 6 (or the given
+ * number of) accepting lanes; "auto" does so in the first 3 sweeps after the
+ * spins were set, when most attempts are accepted; the results are the same
+ * in every mode),
  * "slices_per_block" (int, 0 = auto), "sweep_launch" ("auto" | "cooperative"),
  * "refresh_interval" (int, steps between
  * field recomputes; 0 = only after the spins were set), "refresh_engine"

@@ -376,14 +376,24 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
             clearState(solPrepared);
         }
     } else if (name == "multi_flip") {
-        /* float local-field sweeps: batches that commit every flip of their
-         * 64 attempts in order ("on") or end at the first flip ("off") */
-        if (value == "on" || value == "1" || value == "true" || value == "True")
-            multiFlip_ = true;
-        else if (value == "off" || value == "0" || value == "false" || value == "False")
-            multiFlip_ = false;
-        else
-            SQ_THROW("multi_flip must be on or off");
+        /* float local-field sweeps: "off" (every batch ends at its first
+         * flip), "auto" (multi-flip batches in the first sweeps after the
+         * spins were set), "on" (in every sweep), or a number 2..64: "on"
+         * with that threshold of accepting lanes */
+        if (value == "on" || value == "true" || value == "True") {
+            multiFlipMode_ = kMultiFlipOn;
+            multiFlipMin_ = kMultiFlipDefaultMin;
+        } else if (value == "auto") {
+            multiFlipMode_ = kMultiFlipAuto;
+            multiFlipMin_ = kMultiFlipDefaultMin;
+        } else if (value == "off" || value == "0" || value == "false" || value == "False") {
+            multiFlipMode_ = kMultiFlipOff;
+        } else {
+            const int v = atoi(value.c_str());
+            SQ_THROW_IF(v < 2 || v > 64 || std::to_string(v) != value, "multi_flip must be auto, on, off or 2..64");
+            multiFlipMode_ = kMultiFlipOn;
+            multiFlipMin_ = v;
+        }
     } else if (name == "spin_limit") {
         const int v = atoi(value.c_str());
         SQ_THROW_IF(v <= 0, "spin_limit must be positive");
@@ -416,7 +426,10 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
     p.emplace_back("refresh_interval", std::to_string(refreshInterval_));
     p.emplace_back("refresh_engine", refreshVector_ ? "vector" : "mfma");
     p.emplace_back("profile", std::to_string(profile_));
-    p.emplace_back("multi_flip", multiFlip_ ? "on" : "off");
+    p.emplace_back("multi_flip", multiFlipMode_ == kMultiFlipOff            ? std::string("off")
+                                 : multiFlipMode_ == kMultiFlipAuto         ? std::string("auto")
+                                 : multiFlipMin_ == kMultiFlipDefaultMin    ? std::string("on")
+                                                                            : std::to_string(multiFlipMin_));
     return p;
 }
 
@@ -599,6 +612,7 @@ void DenseGraphAnnealer<real>::randomizeSpin() {
         spinsFromPhilox(seed_, sweepIdx_++, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
     }
     fieldsDirty_ = true;
+    sweepsSinceSpinsSet_ = 0;
     setState(solQSet);
 }
 
@@ -616,6 +630,7 @@ void DenseGraphAnnealer<real>::set_q(const int8_t *q, int N) {
     packSpins(d_q8_.ptr, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
     dev_->synchronize();
     fieldsDirty_ = true;
+    sweepsSinceSpinsSet_ = 0;
     setState(solQSet);
 }
 
@@ -632,6 +647,7 @@ void DenseGraphAnnealer<real>::set_qset(const int8_t *q, int m, int N) {
     packSpins(d_q8_.ptr, N_, m_, Nw_, d_Q_.ptr, dev_->stream());
     dev_->synchronize();
     fieldsDirty_ = true;
+    sweepsSinceSpinsSet_ = 0;
     setState(solQSet);
 }
 
@@ -848,9 +864,13 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
                 "N=%d too large for the LDS-resident sweep (%zu bytes needed per block)", N_,
                 denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_));
     /* multi-flip batches where LDS allows (float local-field forms but fp8) */
-    a.multiBufs = multiFlip_ ? denseSweepMultiBufs(Np_, S, sizeof(real), plainSweep_, fieldFormat_,
-                                                   dev_->maxLdsPerBlock())
-                             : 0;
+    const bool multi = multiFlipMode_ == kMultiFlipOn ||
+                       (multiFlipMode_ == kMultiFlipAuto && sweepsSinceSpinsSet_ < kMultiFlipAutoSweeps);
+    a.multiMinCand = multiFlipMin_;
+    a.multiBufs = multi ? denseSweepMultiBufs(Np_, S, sizeof(real), plainSweep_, fieldFormat_,
+                                              dev_->maxLdsPerBlock())
+                        : 0;
+    ++sweepsSinceSpinsSet_;
     if (profile_)
         phaseBegin(2, pe);
     denseSweep<real>(a, plainSweep_, fieldFormat_, S, !coopLaunch_ && dev_->soleSweeper(), dev_->numCUs(),

@@ -982,10 +982,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         SQ_CYC(r2);
         SQ_CYC_ADD(cyc[10], r1, r2);
         if constexpr (kMultiCap) {
-            /* three or more lanes accepting (flips of a multi batch, or
-             * candidates of a single-flip batch): the next batch resolves
-             * its flips in order instead of ending at the first */
-            multi = nBufs >= 2 && nCand >= 3;
+            /* enough lanes accepting (flips of a multi batch, or candidates
+             * of a single-flip batch): the next batch resolves its flips in
+             * order instead of ending at the first */
+            multi = nBufs >= 2 && nCand >= uni(a.multiMinCand);
         }
         k = uni(k + c);
         waveSync();

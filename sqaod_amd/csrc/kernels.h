@@ -264,7 +264,17 @@ struct SweepArgs {
     int spinLimit;
     int multiBufs;          /* float local-field form: row stage buffers for batches that commit several
                              * flips (>= 2), 0: every batch ends at its first flip */
+    int multiMinCand;       /* with multiBufs >= 2: accepting lanes of a batch that make the next batch
+                             * commit all its flips */
 };
+
+/* multi_flip: "off", "auto" (the default: multi-flip batches in the first
+ * kMultiFlipAutoSweeps sweeps after the spins were set, when most attempts
+ * are accepted), "on" (every sweep), each with the threshold "on" stands
+ * for unless a number is given. */
+enum { kMultiFlipOff = 0, kMultiFlipAuto = 1, kMultiFlipOn = 2 };
+constexpr int kMultiFlipDefaultMin = 6;
+constexpr int kMultiFlipAutoSweeps = 3;
 
 /* One annealOneStep sweep: N attempts per slice with exact serial-coloring
  * semantics.  plain = true computes each attempt's field as a J-row . q-row

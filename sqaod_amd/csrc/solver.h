@@ -227,7 +227,13 @@ private:
     bool refreshVector_ = false;  /* "refresh_engine": "mfma" (default) | "vector" */
     int spinLimit_ = 1 << 20;
     int nWorkers_ = 1;        /* CPU workers whose MT19937 streams the coloring step replays */
-    bool multiFlip_ = false;  /* multi-flip batches in the float local-field sweep */
+    /* multi-flip batches in the float local-field sweep: off, on, or (auto)
+     * in the first kMultiFlipAutoSweeps sweeps after the spins were set;
+     * with them, a batch with at least multiFlipMin_ accepting lanes makes
+     * the next one resolve all its flips in order */
+    int multiFlipMode_ = kMultiFlipAuto;
+    int multiFlipMin_ = kMultiFlipDefaultMin;
+    int sweepsSinceSpinsSet_ = 0;
     int sysTraceCap_ = 0, sysTraceLen_ = 0;
     long long sysTraceSteps_ = 0;
     int mtWorkersSeeded_ = 0; /* generators in d_mt_ */

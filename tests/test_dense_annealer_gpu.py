@@ -315,14 +315,15 @@ def test_generators_continue_across_prepare(oracle, rocm):
                                            (1000, 16, "bf16x2", "coloring"), (96, 1, "fp32", "sa_naive"),
                                            (2048, 64, "fp32", "coloring")])
 def test_multi_flip_batches_bit_exact(oracle, rocm, N, m, form, algo):
-    """multi_flip=on: a batch resolves all its accepted flips in order (each
+    """multi_flip: a batch resolves all its accepted flips in order (each
     flip corrects the later lanes' fields with the gathered submatrix and the
     rows are applied in flip order), so the trajectory is still the serial
     CPU annealer's, bit for bit, in the flip-heavy start of the schedule and
     to its end."""
     W = dense_int_W(N, SEED_BASE + 3 * N + m)
-    ora, an = make_pair(oracle, rocm, W, m, seed=31 + m, algo=algo, field_precision=form, multi_flip="on")
-    assert an.get_preferences()["multi_flip"] == "on"
+    # threshold 2: every batch after one with two accepting lanes is a multi-flip batch
+    ora, an = make_pair(oracle, rocm, W, m, seed=31 + m, algo=algo, field_precision=form, multi_flip="2")
+    assert an.get_preferences()["multi_flip"] == "2"
     sch = schedule() if N <= 300 else schedule()[:40]
     if algo == "sa_naive":
         sch = list(np.geomspace(20.0, 0.05, 60))
