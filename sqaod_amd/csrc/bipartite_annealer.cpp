@@ -165,20 +165,18 @@ void BipartiteGraphAnnealer<real>::uploadProblem(const std::vector<double> &h0, 
 }
 
 template <class real>
-void BipartiteGraphAnnealer<real>::setQUBO(const real *b0, const real *b1, const real *W, int N0, int N1,
-                                           OptimizeMethod om) {
-    /* :55-75 with BGFuncs::calculateHamiltonian (SharedFormulas.cpp:147-167):
-     * J = -1/4 W, h0 = -1/4 colsum W - 1/2 b0, h1 = -1/4 rowsum W - 1/2 b1,
+void BipartiteGraphAnnealer<real>::hamiltonianOf(const real *b0, const real *b1, const real *W, int N0, int N1,
+                                                 OptimizeMethod om, std::vector<double> &h0,
+                                                 std::vector<double> &h1, std::vector<double> &J, double &c) {
+    /* BGFuncs::calculateHamiltonian (SharedFormulas.cpp:147-167): J = -1/4 W,
+     * h0 = -1/4 colsum W - 1/2 b0, h1 = -1/4 rowsum W - 1/2 b1,
      * c = -1/4 sum W - 1/2 (sum b0 + sum b1); all negated for maximize.  Sums
      * are taken in double and rounded once (exact for quarter-integer data). */
     SQ_THROW_IF(N0 <= 0 || N1 <= 0, "N0 and N1 must be positive, N0=%d, N1=%d.", N0, N1);
-    clearState(solProblemSet);
-    N0_ = N0;
-    N1_ = N1;
-    m_ = (N0 + N1) / 4; /* :61 */
-    om_ = om;
     const double sgn = (om == optMaximize) ? -1. : 1.;
-    std::vector<double> h0(N0), h1(N1), J((size_t)N1 * N0);
+    h0.assign(N0, 0.);
+    h1.assign(N1, 0.);
+    J.assign((size_t)N1 * N0, 0.);
     double wsum = 0., b0s = 0., b1s = 0.;
     for (int j = 0; j < N0; ++j) {
         double s = 0.;
@@ -199,8 +197,31 @@ void BipartiteGraphAnnealer<real>::setQUBO(const real *b0, const real *b1, const
         b0s += (double)b0[j];
     for (int i = 0; i < N1; ++i)
         b1s += (double)b1[i];
-    uploadProblem(h0, h1, J, sgn * (-0.25 * wsum - 0.5 * (b0s + b1s)));
+    c = sgn * (-0.25 * wsum - 0.5 * (b0s + b1s));
+}
+
+template <class real>
+void BipartiteGraphAnnealer<real>::setProblem(const std::vector<double> &h0, const std::vector<double> &h1,
+                                              const std::vector<double> &J, double c, int N0, int N1,
+                                              OptimizeMethod om) {
+    SQ_THROW_IF(N0 <= 0 || N1 <= 0, "N0 and N1 must be positive, N0=%d, N1=%d.", N0, N1);
+    clearState(solProblemSet);
+    N0_ = N0;
+    N1_ = N1;
+    m_ = (N0 + N1) / 4; /* :61 */
+    om_ = om;
+    uploadProblem(h0, h1, J, c);
     setState(solProblemSet);
+}
+
+template <class real>
+void BipartiteGraphAnnealer<real>::setQUBO(const real *b0, const real *b1, const real *W, int N0, int N1,
+                                           OptimizeMethod om) {
+    /* :55-75 */
+    std::vector<double> h0, h1, J;
+    double c;
+    hamiltonianOf(b0, b1, W, N0, N1, om, h0, h1, J, c);
+    setProblem(h0, h1, J, c, N0, N1, om);
 }
 
 template <class real>

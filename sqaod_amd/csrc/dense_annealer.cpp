@@ -486,20 +486,20 @@ void DenseGraphAnnealer<real>::prepareFieldForm() {
 }
 
 template <class real>
-void DenseGraphAnnealer<real>::setQUBO(const real *W, int N, OptimizeMethod om) {
-    /* CPUDenseGraphAnnealer.cpp:52-69 with DGFuncs::calculateHamiltonian
-     * (SharedFormulas.cpp:36-55): h = -1/2 colsum W, J = -1/4 W, c = sum J +
-     * trace J, diag J = 0; maximize negates h, J, c.  Sums are accumulated in
+void DenseGraphAnnealer<real>::hamiltonianOf(const real *W, int N, OptimizeMethod om, bool check,
+                                             std::vector<real> &h, std::vector<real> &J, real &c) {
+    /* DGFuncs::calculateHamiltonian (SharedFormulas.cpp:36-55): h = -1/2
+     * colsum W, J = -1/4 W, c = sum J + trace J, diag J = 0; maximize negates
+     * h, J, c (CPUDenseGraphAnnealer.cpp:52-69).  Sums are accumulated in
      * double and rounded once. */
     SQ_THROW_IF(N <= 0, "W must be a non-empty square matrix.");
-    for (int i = 0; i < N; ++i)
-        for (int j = 0; j < i; ++j)
-            SQ_THROW_IF(W[(size_t)i * N + j] != W[(size_t)j * N + i], "W is not symmetric.");
-    clearState(solProblemSet);
-    allocateProblem(N);
-    om_ = om;
+    if (check)
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < i; ++j)
+                SQ_THROW_IF(W[(size_t)i * N + j] != W[(size_t)j * N + i], "W is not symmetric.");
     const real sgn = (om == optMaximize) ? real(-1.) : real(1.);
-    std::vector<real> h(N), J((size_t)N * N);
+    h.assign(N, real(0.));
+    J.assign((size_t)N * N, real(0.));
     double jsum = 0., dsum = 0.;
     for (int j = 0; j < N; ++j) {
         double s = 0.;
@@ -515,8 +515,25 @@ void DenseGraphAnnealer<real>::setQUBO(const real *W, int N, OptimizeMethod om) 
                 dsum += (double)v;
             J[(size_t)i * N + j] = (i == j) ? real(0.) : sgn * v;
         }
-    uploadJ(J.data(), h.data(), sgn * (real)(jsum + dsum));
+    c = sgn * (real)(jsum + dsum);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::setProblem(const real *h, const real *J, real c, int N, OptimizeMethod om) {
+    clearState(solProblemSet);
+    allocateProblem(N);
+    om_ = om;
+    uploadJ(J, h, c);
     setState(solProblemSet);
+}
+
+template <class real>
+void DenseGraphAnnealer<real>::setQUBO(const real *W, int N, OptimizeMethod om) {
+    /* CPUDenseGraphAnnealer.cpp:52-69 */
+    std::vector<real> h, J;
+    real c;
+    hamiltonianOf(W, N, om, true, h, J, c);
+    setProblem(h.data(), J.data(), c, N, om);
 }
 
 template <class real>

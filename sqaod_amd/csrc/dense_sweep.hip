@@ -253,9 +253,14 @@ __host__ __device__ constexpr size_t sweepFixedBytes() {
     return (size_t)4 * kWin * 4 + (size_t)kRing * (4 + sizeof(real)) + 16;
 }
 
-/* Multi-flip batches (float local-field form): the J submatrix of the
- * batch's sites, element [j][i] = J[s_i][s_j] for lanes i > j. */
-constexpr size_t kSubBytes = (size_t)kWave * kWave * 4;
+/* Multi-flip batches (float local-field form) cover the first kMultiLanes
+ * attempts of a batch; the J submatrix of their sites, element [j][i] =
+ * J[s_i][s_j] for lanes i > j, is gathered one scattered dword per element,
+ * so its memory traffic grows with the square of the lane count.  (32 lanes,
+ * a quarter of the traffic, measured no faster than 64: the batches end
+ * earlier.) */
+constexpr int kMultiLanes = kWave;
+constexpr size_t kSubBytes = (size_t)kMultiLanes * kWave * 4;
 
 template <class real, int MODE>
 __host__ __device__ constexpr bool multiFlipCapable() {
@@ -381,6 +386,15 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     bool rowPending = false;
     real rowD = 0;
     int curStage = 0; /* the stage buffer holding the pending row */
+    /* multi-flip kernel: up to nBufs deferred rows, row p in stage buffer p
+     * with factor pendD[p], added to a field in row order (the serial order
+     * of their flips) by the evaluations and, in one pass over F, by the next
+     * commit */
+    constexpr int kMaxPend = 4;
+    int nPend = 0;
+    real pendD[kMaxPend] = {0, 0, 0, 0};
+    (void)nPend;
+    (void)pendD;
     /* F += d * (row in stage buffer b), four 256-element slices per round:
      * all their LDS reads are issued before the first write, so the round
      * trips overlap (Np is a multiple of 256; the tail round covers what is
@@ -420,6 +434,45 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         applyRow(curStage, rowD);
         rowPending = false;
     };
+    /* F += pendD[0] * row 0, then pendD[1] * row 1, ... per element (the
+     * serial order), reading F once for all the deferred rows */
+    auto applyPendingRows = [&]() {
+        if (nPend == 1) { /* one row: the deeper single-row rounds */
+            applyRow(0, pendD[0]);
+            nPend = 0;
+            return;
+        }
+        constexpr int kR = 2;
+        for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
+            real fv[kR][4], sv[kR][kMaxPend][4];
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                const int j = b0 + r * kWave * 4 + lane * 4;
+                if (b0 + r * kWave * 4 < Np) {
+                    ldsLoad4(F + j, fv[r]);
+#pragma unroll
+                    for (int q = 0; q < kMaxPend; ++q)
+                        if (q < nPend)
+                            ldsLoad4(stageBuf(q) + j, sv[r][q]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kR; ++r) {
+                const int j = b0 + r * kWave * 4 + lane * 4;
+                if (b0 + r * kWave * 4 < Np) {
+#pragma unroll
+                    for (int q = 0; q < kMaxPend; ++q)
+                        if (q < nPend)
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                fv[r][t] = fv[r][t] + pendD[q] * sv[r][q][t];
+                    ldsStore4(F + j, fv[r]);
+                }
+            }
+        }
+        nPend = 0;
+    };
+    (void)applyPendingRows;
 
     /* Memory traffic of the batch loop.  What the loop reads from global
      * memory arrives in LDS by DMA (dmaDword), so its memory waits are these,
@@ -501,14 +554,15 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             s = 0u;
         if constexpr (kMultiCap) {
             if (multi) {
-                /* sub[j][i] = J[s_i][s_j] for lanes i > j: 63 DMA ops of 64
-                 * scattered dwords, landing while the neighbour phase runs
-                 * (waited for with the staged row below) */
+                /* sub[j][i] = J[s_i][s_j] for lanes j < i < kMultiLanes:
+                 * kMultiLanes - 1 DMA ops of scattered dwords, landing while
+                 * the neighbour phase runs (waited for with the staged row
+                 * below) */
                 ldsWait(); /* this wave's reads of sub (last batch) are done */
                 const real *myRow = a.J + (long long)s * Np;
-                for (int j = 0; j < kWave - 1; ++j) {
+                for (int j = 0; j < kMultiLanes - 1; ++j) {
                     const uint32_t sj = (uint32_t)__builtin_amdgcn_readlane((int)s, j);
-                    if (lane > j)
+                    if (lane > j && lane < kMultiLanes)
                         dmaDword<false>(myRow + sj, subL + (uint32_t)j * (kWave * 4));
                 }
             }
@@ -611,7 +665,17 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         /* the staged row has had the whole neighbour phase to arrive: wait for
          * it before the next windows are requested (vmcnt counts in order) */
         real rowCorr = 0;
-        if constexpr (kDefer) {
+        real pendCorr[kMaxPend] = {0, 0, 0, 0};
+        (void)pendCorr;
+        if constexpr (kMultiCap) {
+            if (nPend > 0 || multi) {
+                vmWaitImm<0>();
+#pragma unroll
+                for (int q = 0; q < kMaxPend; ++q)
+                    if (q < nPend)
+                        pendCorr[q] = stageBuf(q)[s];
+            }
+        } else if constexpr (kDefer) {
             if (rowPending || multi) {
                 vmWaitImm<0>();
                 if (rowPending) {
@@ -690,9 +754,15 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             /* decided under both outcomes of the pending attempt: if they
              * agree the decision is final, otherwise the lane waits */
             real f = F[s];
-            if constexpr (kDefer)
+            if constexpr (kMultiCap) {
+#pragma unroll
+                for (int q = 0; q < kMaxPend; ++q)
+                    if (q < nPend)
+                        f = f + pendD[q] * pendCorr[q];
+            } else if constexpr (kDefer) {
                 if (rowPending)
                     f = f + rowD * rowCorr;
+            }
             accLocal = decide(f);
             if (hypNb >= 0 && decideWith(f, nsumAlt) != accLocal)
                 blocked = true;
@@ -758,12 +828,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
              * later attempt at site x sees the flipped spin, and their
              * decisions and two-outcome checks are redone.  Lanes before the
              * next accepted one are confirmed rejections; the batch ends at
-             * the first lane that is blocked or past the window. */
+             * the first lane that is blocked, past the window or past the
+             * submatrix (kMultiLanes). */
             int j = 0;
             c = kWave;
             for (;;) {
                 const bool act = lane >= j;
-                const int fs = uni(firstLane(__ballot(act && (!valid || blocked))));
+                const int fs = uni(firstLane(__ballot(act && (!valid || blocked || lane >= kMultiLanes))));
                 const int fa = uni(firstLane(__ballot(act && valid && !blocked && accLocal)));
                 if (fa >= fs) {
                     c = fs;
@@ -876,9 +947,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         afterGroup += 1;
 
         SQ_CYC(c4);
-        if constexpr (kDefer)
+        if constexpr (kMultiCap) {
+            if (nPend > 0)
+                applyPendingRows();
+        } else if constexpr (kDefer) {
             if (rowPending)
                 applyPendingRow();
+        }
         SQ_CYC(r1);
         SQ_CYC_ADD(cyc[9], c4, r1);
         if (flipLane >= 0) {
@@ -905,19 +980,22 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                     dmaDword<false>(a.rowScale + x, scaleL);
                 afterGroup += rowOps;
                 rowPending = true;
-            } else if constexpr (kDefer) {
+            } else if constexpr (kMultiCap) {
                 /* F[y][:] += -2 q_x J[x][:] (J symmetric) for the flips in lane
-                 * order.  Rows stream into the stage buffers (1 KiB, 16 B per
-                 * lane, per op), nBufs in flight; every row but the last is
-                 * applied to F here, the last is deferred: the next batch's
-                 * lanes add rowD * stage[s] to F[s] themselves and the next
-                 * commit applies it.  Row r is in buffer r % nBufs; a buffer is
-                 * refilled only after its row was applied, and vmcnt retires in
-                 * issue order, so row r has landed once at most (issued - 1 -
-                 * r) rows' ops are outstanding ((nBufs - 1) rows' ops fit the
-                 * counter). */
+                 * order.  The last min(nFl, nBufs) rows are deferred: they
+                 * stream into stage buffers 0, 1, ..., the next batch's lanes
+                 * add them to their fields in that order, and the next commit
+                 * applies them in one pass.  Earlier rows (batches with more
+                 * flips than buffers) stream through the buffers and are
+                 * applied here one by one: row r is in buffer r % nBufs, a
+                 * buffer is refilled only after its row was applied, and
+                 * vmcnt retires in issue order, so row r has landed once at
+                 * most (issued - 1 - r) rows' ops are outstanding ((nBufs - 1)
+                 * rows' ops fit the counter). */
                 waveSync();
                 ldsWait(); /* this wave's reads of the stage buffers are done */
+                const int nDefer = nFl < nBufs ? nFl : nBufs;
+                const int nImm = nFl - nDefer;
                 unsigned long long issueMask = flipMask, applyMask = flipMask;
                 /* buffer indices wrap by compare (r % nBufs without a divide) */
                 int issued = 0, applied = 0, issueBuf = 0, applyBuf = 0;
@@ -933,22 +1011,42 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                     ++issued;
                     issueBuf = (issueBuf + 1 == nBufs) ? 0 : issueBuf + 1;
                 };
-                while (issued < nFl && issued < nBufs)
+                while (issued < nImm && issued < nBufs)
                     issueNext();
-                while (applied < nFl - 1) {
+                while (applied < nImm) {
                     vmWait((issued - 1 - applied) * rowOps);
                     const int fl = uni(firstLane(applyMask));
                     applyMask &= applyMask - 1;
                     applyRow(applyBuf, readLaneF(dFlip, fl));
                     ++applied;
                     applyBuf = (applyBuf + 1 == nBufs) ? 0 : applyBuf + 1;
-                    if (issued < nFl) {
+                    if (issued < nImm) {
                         ldsWait(); /* the applied buffer's reads are done */
                         issueNext();
                     }
                 }
-                rowD = readLaneF(dFlip, uni(firstLane(applyMask)));
-                curStage = applyBuf; /* the last row's buffer */
+                ldsWait(); /* the applied buffers' reads are done */
+                issueBuf = 0;
+#pragma unroll
+                for (int q = 0; q < kMaxPend; ++q) {
+                    if (q < nDefer) {
+                        const int fl = uni(firstLane(applyMask));
+                        applyMask &= applyMask - 1;
+                        pendD[q] = readLaneF(dFlip, fl);
+                        issueNext();
+                    }
+                }
+                nPend = nDefer;
+            } else if constexpr (kDefer) {
+                /* F[y][:] += -2 q_x J[x][:] (J symmetric), deferred: the row
+                 * streams into `stage`, 1 KiB (16 B per lane) per op */
+                rowD = qx ? (real)-2. : (real)2.;
+                const real *row = a.J + (long long)x * Np;
+                waveSync();
+                ldsWait(); /* this wave's reads of `stage` are done */
+                for (int j0 = 0; j0 < Np; j0 += kWave * 4)
+                    dmaDwordx4(row + j0 + lane * 4, stageL + j0 * 4);
+                afterGroup += rowOps;
                 rowPending = true;
             } else if constexpr (!PLAIN) {
                 /* F[y][:] += -2 q_x J[:][x] = -2 q_x J[x][:]  (J symmetric);
@@ -996,7 +1094,12 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
 
     /* nothing may land in this region after the wave leaves it */
     vmWaitImm<0>();
-    if constexpr (kDefer) {
+    if constexpr (kMultiCap) {
+        if (nPend > 0 && !aborted) {
+            applyPendingRows();
+            waveSync();
+        }
+    } else if constexpr (kDefer) {
         if (rowPending && !aborted) {
             if constexpr (FP8)
                 rowD = rowSign * scaleSlot[0];

@@ -16,12 +16,18 @@
  *  - a job whose schedule is done is finished, its device memory is freed
  *    and the next pending jobs can be admitted at the following tick.
  *
+ * A job's host preparation (its Hamiltonian, O(N^2) on the host) runs on a
+ * thread of the server in submission order; admission waits for it only
+ * when it has not finished, so a large job's preparation does not stall the
+ * ticks of the jobs already running.
+ *
  * Only one persistent sweep runs at a time (size classes are serialised on
  * one stream): two persistent launches that were each partly resident could
  * wait on each other's resources.  Bipartite kernels never wait, so they can
  * share the device with it.
  */
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 
@@ -32,7 +38,14 @@ namespace sqaod_rocm {
 struct Server::Job {
     int id = 0, kind = kDense, N0 = 0, N1 = 0, m = 0;
     unsigned long long seed = 0;
-    std::vector<float> W, b0, b1; /* host copies until admission */
+    std::vector<float> W, b0, b1; /* host copies until the preparation */
+    /* the preparation thread's result: 0 queued, 1 ready, 2 failed */
+    std::atomic<int> prep{0};
+    std::string prepError;
+    std::vector<float> h, J; /* dense Hamiltonian */
+    float c = 0.f;
+    std::vector<double> h0d, h1d, Jd; /* bipartite Hamiltonian */
+    double cd = 0.;
     std::vector<double> G, beta;
     int step = 0;
     size_t bytes = 0;
@@ -72,11 +85,60 @@ Server::Server(Device &dev) : dev_(&dev) {
     t0_ = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     for (int i = 0; i < 4; ++i)
         pool_.emplace_back(new Device(dev.devno()));
+    prepThread_ = std::thread([this] { prepLoop(); });
 }
 
 Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(prepMu_);
+        prepStop_ = true;
+    }
+    prepCv_.notify_all();
+    prepThread_.join();
     active_.clear();
     pending_.clear();
+}
+
+/* The preparation thread: each queued job's Hamiltonian (the host half of
+ * setQUBO), in submission order.  A job stays in pending_ (owned by the
+ * server thread) until admission, which takes it only after prep != 0, so
+ * the job outlives its preparation. */
+void Server::prepLoop() {
+    for (;;) {
+        Job *j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(prepMu_);
+            prepCv_.wait(lk, [this] { return prepStop_ || !prepQueue_.empty(); });
+            if (prepStop_)
+                return;
+            j = prepQueue_.front();
+            prepQueue_.pop_front();
+        }
+        try {
+            if (j->kind == kDense) {
+                /* W was checked for symmetry at submission */
+                DenseGraphAnnealer<float>::hamiltonianOf(j->W.data(), j->N0, optMinimize, false, j->h, j->J, j->c);
+            } else {
+                BipartiteGraphAnnealer<float>::hamiltonianOf(j->b0.data(), j->b1.data(), j->W.data(), j->N0, j->N1,
+                                                             optMinimize, j->h0d, j->h1d, j->Jd, j->cd);
+            }
+            std::vector<float>().swap(j->W);
+            std::vector<float>().swap(j->b0);
+            std::vector<float>().swap(j->b1);
+            j->prep.store(1, std::memory_order_release);
+        } catch (const std::exception &e) {
+            j->prepError = e.what();
+            j->prep.store(2, std::memory_order_release);
+        }
+    }
+}
+
+void Server::enqueuePrep(Job *j) {
+    {
+        std::lock_guard<std::mutex> lk(prepMu_);
+        prepQueue_.push_back(j);
+    }
+    prepCv_.notify_one();
 }
 
 double Server::now() const {
@@ -171,6 +233,7 @@ int Server::submitDense(const float *W, int N, int m, unsigned long long seed, c
                 "dense job N=%d, m=%d does not fit the grouped sweep (LDS region per slice or resident waves)", N,
                 m);
     j->submitS = now();
+    enqueuePrep(j.get());
     pending_.push_back(std::move(j));
     return pending_.back()->id;
 }
@@ -193,6 +256,7 @@ int Server::submitBipartite(const float *b0, const float *b1, const float *W, in
     j->bytes = estimateBipartiteBytes(N0, N1, m, bipartiteFormat_);
     SQ_THROW_IF(j->bytes > budget_, "job needs ~%zu MiB, over the %zu MiB budget", j->bytes >> 20, budget_ >> 20);
     j->submitS = now();
+    enqueuePrep(j.get());
     pending_.push_back(std::move(j));
     return pending_.back()->id;
 }
@@ -210,17 +274,24 @@ bool Server::denseGroupFits(int maxNp, int maxLd8, int maxM) const {
 void Server::admit() {
     while (!pending_.empty()) {
         Job &j = *pending_.front();
-        if ((int)active_.size() >= maxActive_ || memInUse_ + j.bytes > budget_)
+        const int prep = j.prep.load(std::memory_order_acquire);
+        if (prep == 0)
+            break; /* arrival order: the head job's Hamiltonian is still being formed */
+        if (prep == 1 && ((int)active_.size() >= maxActive_ || memInUse_ + j.bytes > budget_))
             break; /* arrival order: a job that does not fit holds the ones behind it */
         std::unique_ptr<Job> job = std::move(pending_.front());
         pending_.erase(pending_.begin());
+        if (prep == 2) {
+            fail(j, j.prepError.c_str());
+            continue;
+        }
         try {
             if (j.kind == kDense) {
                 j.dense.reset(new DenseGraphAnnealer<float>());
                 auto &a = *j.dense;
                 a.assignDevice(*dev_);
                 a.seed(j.seed);
-                a.setQUBO(j.W.data(), j.N0, optMinimize);
+                a.setProblem(j.h.data(), j.J.data(), j.c, j.N0, optMinimize);
                 a.setPreference("n_trotters", std::to_string(j.m));
                 a.setPreference("field_precision", formatName(denseFormat_));
                 if (denseSpinLimit_ > 0)
@@ -233,7 +304,7 @@ void Server::admit() {
                 a.assignDevice(*pool_[nextStream_]);
                 nextStream_ = (nextStream_ + 1) % (int)pool_.size();
                 a.seed(j.seed);
-                a.setQUBO(j.b0.data(), j.b1.data(), j.W.data(), j.N0, j.N1, optMinimize);
+                a.setProblem(j.h0d, j.h1d, j.Jd, j.cd, j.N0, j.N1, optMinimize);
                 a.setPreference("n_trotters", std::to_string(j.m));
                 a.setPreference("field_precision", formatName(bipartiteFormat_));
                 a.prepare();
@@ -243,9 +314,11 @@ void Server::admit() {
             fail(j, e.what()); /* device state (if any) is freed with the job */
             continue;
         }
-        std::vector<float>().swap(j.W);
-        std::vector<float>().swap(j.b0);
-        std::vector<float>().swap(j.b1);
+        std::vector<float>().swap(j.h);
+        std::vector<float>().swap(j.J);
+        std::vector<double>().swap(j.h0d);
+        std::vector<double>().swap(j.h1d);
+        std::vector<double>().swap(j.Jd);
         j.admitS = now();
         j.admitted = true;
         memInUse_ += j.bytes;

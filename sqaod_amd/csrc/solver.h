@@ -9,8 +9,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstdint>
+#include <deque>
 #include <functional>
+#include <mutex>
+#include <thread>
 #include <map>
 #include <memory>
 #include <string>
@@ -122,6 +126,12 @@ public:
 
     void setQUBO(const real *W, int N, OptimizeMethod om);
     void setHamiltonian(const real *h, const real *J, real c, int N);
+    /* setQUBO in two halves: the host Hamiltonian (h, J N x N with a zero
+     * diagonal, c; a pure function, callable from any thread; check = test W
+     * for symmetry) and its upload as the annealer's problem */
+    static void hamiltonianOf(const real *W, int N, OptimizeMethod om, bool check, std::vector<real> &h,
+                              std::vector<real> &J, real &c);
+    void setProblem(const real *h, const real *J, real c, int N, OptimizeMethod om);
     void getHamiltonian(real *h, real *J, real *c) const;
     int getProblemSize() const { return N_; }
     int numTrotters() const { return m_; }
@@ -490,6 +500,12 @@ public:
     /* W is N1 x N0 (rows: the x1 side), b0[N0], b1[N1] */
     void setQUBO(const real *b0, const real *b1, const real *W, int N0, int N1, OptimizeMethod om);
     void setHamiltonian(const real *h0, const real *h1, const real *J, real c, int N0, int N1);
+    /* setQUBO in two halves: the host Hamiltonian in double (a pure
+     * function, callable from any thread) and its upload as the problem */
+    static void hamiltonianOf(const real *b0, const real *b1, const real *W, int N0, int N1, OptimizeMethod om,
+                              std::vector<double> &h0, std::vector<double> &h1, std::vector<double> &J, double &c);
+    void setProblem(const std::vector<double> &h0, const std::vector<double> &h1, const std::vector<double> &J,
+                    double c, int N0, int N1, OptimizeMethod om);
     void getHamiltonian(real *h0, real *h1, real *J, real *c) const;
     void getProblemSize(int *N0, int *N1) const;
     int numTrotters() const { return m_; }
@@ -635,6 +651,8 @@ private:
     void stepDenseGroup(const std::vector<DenseGraphAnnealer<float> *> &dense, const std::vector<Job *> &jobs,
                         const std::vector<int> &order, GroupedDenseStepper &stepper);
     bool denseGroupFits(int maxNp, int maxLd8, int maxM) const;
+    void prepLoop();
+    void enqueuePrep(Job *j);
 
     Device *dev_;
     std::vector<std::unique_ptr<Device>> pool_;
@@ -650,6 +668,13 @@ private:
     std::vector<int> newlyDone_;
     std::map<int, GroupedDenseStepper> steppers_; /* per size class */
     double t0_;
+    /* host preparation (the jobs' Hamiltonians) on a thread of its own, in
+     * submission order, so that a tick does not wait on it */
+    std::mutex prepMu_;
+    std::condition_variable prepCv_;
+    std::deque<Job *> prepQueue_;
+    bool prepStop_ = false;
+    std::thread prepThread_;
 };
 
 }  // namespace sqaod_rocm

@@ -11,9 +11,14 @@ Annealer: the oracle's restatement of sqaod.cpu's OpenMP bipartite annealer
 (CPUBipartiteGraphAnnealer.cpp:331-373, the coloring half-steps with the
 slices of a colour split in static chunks over 8 workers, worker w drawing
 from its own MT19937 seeded s + 17 w, CPUBipartiteGraphAnnealer.cpp:25-33)
-with annealer seeds 1000 .. 1999, one anneal per seed: seed, randomize_spin,
-205 steps, then E of every slice (CPUBipartiteGraphAnnealer.cpp:215-222).
-Each seed contributes min_y E_y and mean_y E_y.
+with the annealer seeds 1000 + 136 i, i = 0 .. 999, one anneal per seed: seed,
+randomize_spin, 205 steps, then E of every slice
+(CPUBipartiteGraphAnnealer.cpp:215-222).  Each seed contributes min_y E_y and
+mean_y E_y.  The seeds are 136 = 8 x 17 apart because worker w of seed s
+draws from the generator seeded s + 17 w: with consecutive seeds, seed s's
+worker w + 1 replays seed s + 17's worker w, and the energies of seeds 17
+apart correlate (0.6 measured), which breaks the independence the KS test
+assumes.
 
 About 16 s per seed on one core (40 minutes with 6 worker processes):
     KS_PROCS=6 python tests/golden/make_bipartite_ks.py
@@ -30,7 +35,7 @@ N0 = N1 = 1024
 M = 64
 BETA = 50.0
 N_WORKERS = 8
-SEEDS = range(1000, 2000)
+SEEDS = [1000 + 8 * 17 * i for i in range(1000)]  # no two runs share a worker stream
 PROBLEM_SEED = 1133557
 
 
