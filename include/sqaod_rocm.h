@@ -387,7 +387,7 @@ int sqaod_rocm_server_new(sqaod_rocm_server **out);
 int sqaod_rocm_server_delete(sqaod_rocm_server *s);
 /* memory_budget_mb, max_active, dense_field_precision (fp32 | bf16x2 | fp8),
  * bipartite_field_precision (fp32 | bf16x2), bipartite_streams, block_fraction,
- * dense_spin_limit (the bounded wait of dense jobs admitted afterwards; 0 = default) */
+ * dense_spin_limit (the bounded wait of dense jobs submitted afterwards; 0 = default) */
 int sqaod_rocm_server_set_preference(sqaod_rocm_server *s, const char *name, const char *value);
 /* W: N x N symmetric (dense) or N1 x N0 (bipartite, b0[N0], b1[N1]); minimized.
  * A job over the memory budget, or a dense job whose N or m the grouped sweep

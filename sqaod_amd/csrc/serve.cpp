@@ -46,6 +46,7 @@ struct Server::Job {
     float c = 0.f;
     std::vector<double> h0d, h1d, Jd; /* bipartite Hamiltonian */
     double cd = 0.;
+    int spinLimit = 0; /* dense: the server's dense_spin_limit when submitted */
     std::vector<double> G, beta;
     int step = 0;
     size_t bytes = 0;
@@ -172,7 +173,7 @@ void Server::setPreference(const std::string &name, const std::string &value) {
         blockFraction_ = atof(value.c_str());
         SQ_THROW_IF(!(blockFraction_ > 0. && blockFraction_ <= 1.), "block_fraction must be in (0, 1]");
     } else if (name == "dense_spin_limit") {
-        /* the sweep's bounded wait of the dense jobs admitted from now on
+        /* the sweep's bounded wait of the dense jobs submitted from now on
          * (0: the annealer's default); a tiny limit makes their steps fail */
         denseSpinLimit_ = atoi(value.c_str());
         SQ_THROW_IF(denseSpinLimit_ < 0, "dense_spin_limit must be >= 0");
@@ -224,6 +225,7 @@ int Server::submitDense(const float *W, int N, int m, unsigned long long seed, c
     j->G.assign(G, G + nSteps);
     j->beta.assign(beta, beta + nSteps);
     j->bytes = estimateDenseBytes(N, m, denseFormat_);
+    j->spinLimit = denseSpinLimit_;
     SQ_THROW_IF(j->bytes > budget_, "job needs ~%zu MiB, over the %zu MiB budget", j->bytes >> 20, budget_ >> 20);
     /* the grouped sweep holds a job's fields in one wave's LDS region and
      * needs its m slices resident at once: a job past either limit could
@@ -294,8 +296,8 @@ void Server::admit() {
                 a.setProblem(j.h.data(), j.J.data(), j.c, j.N0, optMinimize);
                 a.setPreference("n_trotters", std::to_string(j.m));
                 a.setPreference("field_precision", formatName(denseFormat_));
-                if (denseSpinLimit_ > 0)
-                    a.setPreference("spin_limit", std::to_string(denseSpinLimit_));
+                if (j.spinLimit > 0)
+                    a.setPreference("spin_limit", std::to_string(j.spinLimit));
                 a.prepare();
                 a.randomizeSpin();
             } else {
