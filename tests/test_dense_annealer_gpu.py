@@ -313,13 +313,16 @@ def test_generators_continue_across_prepare(oracle, rocm):
 @pytest.mark.parametrize("N,m,form,algo", [(128, 32, "fp32", "coloring"), (128, 32, "bf16x2", "coloring"),
                                            (300, 7, "fp32", "coloring"), (64, 2, "fp32", "coloring"),
                                            (1000, 16, "bf16x2", "coloring"), (96, 1, "fp32", "sa_naive"),
-                                           (2048, 64, "fp32", "coloring")])
+                                           (2048, 64, "fp32", "coloring"), (5000, 4, "fp32", "coloring"),
+                                           (6000, 3, "bf16x2", "coloring")])
 def test_multi_flip_batches_bit_exact(oracle, rocm, N, m, form, algo):
     """multi_flip: a batch resolves all its accepted flips in order (each
     flip corrects the later lanes' fields with the gathered submatrix and the
     rows are applied in flip order), so the trajectory is still the serial
     CPU annealer's, bit for bit, in the flip-heavy start of the schedule and
-    to its end."""
+    to its end.  N = 5000 and 6000 leave LDS for three and two row stage
+    buffers (four at N <= 4096), so batches with more flips than buffers
+    apply their earlier rows at once and defer fewer."""
     W = dense_int_W(N, SEED_BASE + 3 * N + m)
     # threshold 2: every batch after one with two accepting lanes is a multi-flip batch
     ora, an = make_pair(oracle, rocm, W, m, seed=31 + m, algo=algo, field_precision=form, multi_flip="2")
@@ -337,3 +340,24 @@ def test_multi_flip_batches_bit_exact(oracle, rocm, N, m, form, algo):
     assert np.array_equal(an.get_E(), ora.calculate_E())
     c = an.get_counters()
     assert c["flips"] > 0
+
+
+@pytest.mark.parametrize("N,m", [(5000, 4), (6000, 3), (2048, 16)])
+def test_multi_flip_engages_and_matches_single_flip(rocm, N, m):
+    """The multi-flip sweep takes over where LDS holds at least two row stage
+    buffers (four at N = 2048, three at 5000, two at 6000): in the flip-heavy
+    first steps it commits the same spins as the single-flip sweep in fewer
+    batches."""
+    W = dense_int_W(N, SEED_BASE + 7 * N)
+    runs = {}
+    for mode in ("off", "2"):
+        an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m, multi_flip=mode)
+        an.seed(5)
+        an.prepare()
+        an.randomize_spin()
+        for G in schedule()[:3]:
+            an.anneal_one_step(G, BETA)
+        runs[mode] = (q_of(an), an.get_counters())
+    assert np.array_equal(runs["off"][0], runs["2"][0])
+    assert runs["2"][1]["flips"] == runs["off"][1]["flips"]
+    assert runs["2"][1]["batches"] < 0.8 * runs["off"][1]["batches"]
