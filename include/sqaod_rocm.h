@@ -84,7 +84,7 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
  * "sweep" ("local" | "plain"), "field_precision" ("fp32" | "bf16x2" | "fp8"),
  * "multi_flip" ("auto" | "on" | "off" | a number 2..64: float local-field
  * sweeps commit every flip of a 64-attempt batch in order, instead of ending
- * the batch at its first flip, after a batch with at least 6 (or the given
+ * the batch at its first flip, after a batch with at least 4 (or the given
  * number of) accepting lanes; "auto" does so in the first 3 sweeps after the
  * spins were set, when most attempts are accepted; the results are the same
  * in every mode),

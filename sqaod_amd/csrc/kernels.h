@@ -273,7 +273,7 @@ struct SweepArgs {
  * are accepted), "on" (every sweep), each with the threshold "on" stands
  * for unless a number is given. */
 enum { kMultiFlipOff = 0, kMultiFlipAuto = 1, kMultiFlipOn = 2 };
-constexpr int kMultiFlipDefaultMin = 6;
+constexpr int kMultiFlipDefaultMin = 4;
 constexpr int kMultiFlipAutoSweeps = 3;
 
 /* One annealOneStep sweep: N attempts per slice with exact serial-coloring
