@@ -16,10 +16,11 @@
  *  - a job whose schedule is done is finished, its device memory is freed
  *    and the next pending jobs can be admitted at the following tick.
  *
- * A job's host preparation (its Hamiltonian, O(N^2) on the host) runs on a
- * thread of the server in submission order; admission waits for it only
- * when it has not finished, so a large job's preparation does not stall the
- * ticks of the jobs already running.
+ * A job's host preparation (its Hamiltonian, O(N^2) on the host) runs on one
+ * of a few threads of the server, started in submission order; admission
+ * (still in arrival order) waits for it only when it has not finished, so a
+ * large job's preparation neither stalls the ticks of the jobs already
+ * running nor delays the preparation of the jobs behind it.
  *
  * Only one persistent sweep runs at a time (size classes are serialised on
  * one stream): two persistent launches that were each partly resident could
@@ -72,6 +73,8 @@ int formatOf(const std::string &v) {
 
 size_t roundUp256(int n) { return (size_t)((n + 255) / 256 * 256); }
 
+constexpr int kPrepThreads = 4; /* threads forming the jobs' Hamiltonians */
+
 /* dense jobs share a persistent sweep with jobs of the same padded-size class */
 int sizeClass(int Np) {
     int c = 1024;
@@ -86,7 +89,8 @@ Server::Server(Device &dev) : dev_(&dev) {
     t0_ = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     for (int i = 0; i < 4; ++i)
         pool_.emplace_back(new Device(dev.devno()));
-    prepThread_ = std::thread([this] { prepLoop(); });
+    for (int i = 0; i < kPrepThreads; ++i)
+        prepThreads_.emplace_back([this] { prepLoop(); });
 }
 
 Server::~Server() {
@@ -95,13 +99,14 @@ Server::~Server() {
         prepStop_ = true;
     }
     prepCv_.notify_all();
-    prepThread_.join();
+    for (auto &t : prepThreads_)
+        t.join();
     active_.clear();
     pending_.clear();
 }
 
-/* The preparation thread: each queued job's Hamiltonian (the host half of
- * setQUBO), in submission order.  A job stays in pending_ (owned by the
+/* A preparation thread: the queued jobs' Hamiltonians (the host half of
+ * setQUBO), taken in submission order.  A job stays in pending_ (owned by the
  * server thread) until admission, which takes it only after prep != 0, so
  * the job outlives its preparation. */
 void Server::prepLoop() {

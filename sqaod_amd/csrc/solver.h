@@ -668,13 +668,14 @@ private:
     std::vector<int> newlyDone_;
     std::map<int, GroupedDenseStepper> steppers_; /* per size class */
     double t0_;
-    /* host preparation (the jobs' Hamiltonians) on a thread of its own, in
-     * submission order, so that a tick does not wait on it */
+    /* host preparation (the jobs' Hamiltonians) on threads of their own,
+     * started in submission order, so that a tick does not wait on it and
+     * small jobs are not held behind a large one's preparation */
     std::mutex prepMu_;
     std::condition_variable prepCv_;
     std::deque<Job *> prepQueue_;
     bool prepStop_ = false;
-    std::thread prepThread_;
+    std::vector<std::thread> prepThreads_;
 };
 
 }  // namespace sqaod_rocm
