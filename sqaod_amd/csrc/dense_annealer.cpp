@@ -880,17 +880,24 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
     SQ_THROW_IF(denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_) > dev_->maxLdsPerBlock(),
                 "N=%d too large for the LDS-resident sweep (%zu bytes needed per block)", N_,
                 denseSweepLdsBytes(Np_, S, sizeof(real), plainSweep_, fieldFormat_, ld8_));
-    /* multi-flip batches where LDS allows (float local-field forms but fp8) */
+    /* multi-flip batches where LDS allows (float local-field sweeps).  The
+     * fp8 form's run the float kernel on the float copy of J' (a.J): its row
+     * updates add the same values, -+2 J'[x][s], exactly. */
     const bool multi = multiFlipMode_ == kMultiFlipOn ||
                        (multiFlipMode_ == kMultiFlipAuto && sweepsSinceSpinsSet_ < kMultiFlipAutoSweeps);
+    int sweepFormat = fieldFormat_;
     a.multiMinCand = multiFlipMin_;
-    a.multiBufs = multi ? denseSweepMultiBufs(Np_, S, sizeof(real), plainSweep_, fieldFormat_,
-                                              dev_->maxLdsPerBlock())
-                        : 0;
+    a.multiBufs = 0;
+    if (multi) {
+        const int f = fieldFormat_ == fieldFP8 ? fieldF32 : fieldFormat_;
+        a.multiBufs = denseSweepMultiBufs(Np_, S, sizeof(real), plainSweep_, f, dev_->maxLdsPerBlock());
+        if (a.multiBufs >= 2)
+            sweepFormat = f;
+    }
     ++sweepsSinceSpinsSet_;
     if (profile_)
         phaseBegin(2, pe);
-    denseSweep<real>(a, plainSweep_, fieldFormat_, S, !coopLaunch_ && dev_->soleSweeper(), dev_->numCUs(),
+    denseSweep<real>(a, plainSweep_, sweepFormat, S, !coopLaunch_ && dev_->soleSweeper(), dev_->numCUs(),
                      dev_->stream());
     if (profile_)
         phaseEnd(pe);

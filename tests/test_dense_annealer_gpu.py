@@ -311,6 +311,7 @@ def test_generators_continue_across_prepare(oracle, rocm):
 
 
 @pytest.mark.parametrize("N,m,form,algo", [(128, 32, "fp32", "coloring"), (128, 32, "bf16x2", "coloring"),
+                                           (128, 32, "fp8", "coloring"),
                                            (300, 7, "fp32", "coloring"), (64, 2, "fp32", "coloring"),
                                            (1000, 16, "bf16x2", "coloring"), (96, 1, "fp32", "sa_naive"),
                                            (2048, 64, "fp32", "coloring"), (5000, 4, "fp32", "coloring"),
