@@ -93,11 +93,13 @@ def bytes_per_attempt(N, form):
     return N * bJ + N * 1 + 8
 
 
-def run_form(rocm, W, m, form, steps, warmup, seed, multi_flip=None):
+def run_form(rocm, W, m, form, steps, warmup, seed, multi_flip=None, helper_wave=None):
     N = W.shape[0]
     prefs = dict(n_trotters=m, profile=1)
     if multi_flip is not None and form in ("fp32", "bf16x2"):
         prefs.update(multi_flip=multi_flip)
+    if helper_wave is not None and form != "plain":
+        prefs.update(helper_wave=helper_wave)
     if form == "plain":
         prefs.update(sweep="plain")
     else:
@@ -409,6 +411,8 @@ def main():
     ap.add_argument("--multi-flip", default=None,
                     help="multi-flip batches of the float local-field forms: auto, on, off or the "
                          "threshold 2..64 (default: the library's, auto)")
+    ap.add_argument("--helper-wave", default=None, choices=["on", "off"],
+                    help="second wave per slice applying the deferred rows (default: the library's, on)")
     ap.add_argument("--no-extras", action="store_true", help="skip the bipartite, BF search and serving runs")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--batch-instances", type=int, default=512)
@@ -429,7 +433,8 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     for form in forms:
-        results[form] = run_form(rocm, W, m, form, args.steps, args.warmup, seed=1000, multi_flip=args.multi_flip)
+        results[form] = run_form(rocm, W, m, form, args.steps, args.warmup, seed=1000, multi_flip=args.multi_flip,
+                                 helper_wave=args.helper_wave)
         print(f"# {form}: {results[form]['attempts_per_s']:.4g} attempts/s, "
               f"{results[form]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
 

@@ -394,6 +394,16 @@ void DenseGraphAnnealer<real>::setPreference(const std::string &name, const std:
             multiFlipMode_ = kMultiFlipOn;
             multiFlipMin_ = v;
         }
+    } else if (name == "helper_wave") {
+        /* "on" (default): single-flip float local-field sweeps with one slice
+         * per block run a second wave per slice that applies the deferred J
+         * rows to the fields; "off": the sweeping wave applies them */
+        if (value == "on" || value == "true" || value == "True" || value == "1")
+            helperWave_ = true;
+        else if (value == "off" || value == "false" || value == "False" || value == "0")
+            helperWave_ = false;
+        else
+            SQ_THROW("helper_wave must be on or off");
     } else if (name == "spin_limit") {
         const int v = atoi(value.c_str());
         SQ_THROW_IF(v <= 0, "spin_limit must be positive");
@@ -430,6 +440,7 @@ PrefList DenseGraphAnnealer<real>::getPreferences() const {
                                  : multiFlipMode_ == kMultiFlipAuto         ? std::string("auto")
                                  : multiFlipMin_ == kMultiFlipDefaultMin    ? std::string("on")
                                                                             : std::to_string(multiFlipMin_));
+    p.emplace_back("helper_wave", helperWave_ ? "on" : "off");
     return p;
 }
 
@@ -894,6 +905,10 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
         if (a.multiBufs >= 2)
             sweepFormat = f;
     }
+    a.helperWave = (helperWave_ && a.multiBufs < 2 &&
+                    denseSweepHelperFits(Np_, S, sizeof(real), plainSweep_, sweepFormat, dev_->maxLdsPerBlock()))
+                       ? 1
+                       : 0;
     ++sweepsSinceSpinsSet_;
     if (profile_)
         phaseBegin(2, pe);

@@ -267,12 +267,22 @@ __host__ __device__ constexpr bool multiFlipCapable() {
     return MODE == kModeLocal && sizeof(real) == 4;
 }
 
+/* Helper wave (float local-field sweep, one slice per workgroup): a second
+ * row stage buffer and the command word the two waves exchange. */
+constexpr size_t kHelperCtlBytes = 256;
+
+template <class real, int MODE>
+__host__ __device__ constexpr bool helperCapable() {
+    return MODE == kModeLocal && sizeof(real) == 4;
+}
+
 /* Bytes of LDS one wave's slice needs (the fixed part, fields, row stage,
  * pending table, spin bits and the two mirrors), for a row length Np; with
  * multiBufs >= 2 also multiBufs - 1 more row stage buffers and the
- * submatrix of multi-flip batches. */
+ * submatrix of multi-flip batches; with helper (helperCapable forms) a
+ * second row stage buffer and the helper's command word. */
 template <class real, int MODE>
-__host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8, int multiBufs = 0) {
+__host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8, int multiBufs = 0, bool helper = false) {
     constexpr bool PLAIN = MODE == kModePlain;
     constexpr bool FP8 = MODE == kModeFP8;
     constexpr bool kDefer = !PLAIN && sizeof(real) == 4;
@@ -281,14 +291,110 @@ __host__ __device__ inline size_t sweepWaveBytes(int Np, int ld8, int multiBufs 
     size_t b = sweepFixedBytes<real>() + fieldBytes + (size_t)pendingTableSize(Np) * 4 + (size_t)3 * (Np / 32) * 4;
     if (multiFlipCapable<real, MODE>() && multiBufs >= 2)
         b += (size_t)(multiBufs - 1) * stageBytes + kSubBytes;
+    else if (helperCapable<real, MODE>() && helper)
+        b += stageBytes + kHelperCtlBytes;
     return b;
+}
+
+/* Helper-wave commands (the word at the end of the slice's region): the
+ * sweeping wave writes "apply buffer b with factor -+2" once the row has
+ * landed in LDS (its own vmcnt wait); the helper adds the row to F, waits for
+ * its LDS stores and writes kHelpIdle; kHelpExit ends it.  The word is read
+ * and written with relaxed LDS atomics: a wave's LDS operations execute in
+ * issue order, and the compiler barriers around them keep the field accesses
+ * on their side of the exchange. */
+constexpr uint32_t kHelpIdle = 0, kHelpExit = 1;
+__device__ __forceinline__ uint32_t helpApply(int buf, bool negative) {
+    return 2u + ((uint32_t)buf << 1) + (negative ? 1u : 0u);
+}
+__device__ __forceinline__ uint32_t helpLoad(const uint32_t *ctl) {
+    const uint32_t v = __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    asm volatile("" ::: "memory");
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ void helpStore(uint32_t *ctl, uint32_t v) {
+    asm volatile("" ::: "memory");
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        __hip_atomic_store(ctl, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+/* the slice region's helper command word: after the second stage buffer,
+ * which follows the spin bits and mirrors */
+template <class real>
+__device__ __forceinline__ uint32_t *helperCtl(const SweepArgs<real> &a, unsigned char *p) {
+    const size_t stageBytes = (size_t)a.Np * sizeof(real);
+    unsigned char *body = p + sweepFixedBytes<real>();
+    unsigned char *bits = body + (size_t)a.Np * sizeof(real) + stageBytes + (size_t)pendingTableSize(a.Np) * 4;
+    unsigned char *buf1 = bits + (size_t)3 * a.Nw * 4;
+    return reinterpret_cast<uint32_t *>(buf1 + stageBytes);
+}
+
+/* F += d * (staged row sb), four 256-element slices per round: all their
+ * LDS reads are issued before the first write, so the round trips overlap
+ * (Np is a multiple of 256; the tail round covers what is left).  FP8: sb
+ * holds e4m3 bytes. */
+template <class real, bool FP8>
+__device__ __forceinline__ void addStagedRow(real *F, const real *sb, real d, int Np, int lane) {
+#pragma clang fp contract(off)
+    constexpr int kR = 4;
+    const uint8_t *sb8 = reinterpret_cast<const uint8_t *>(sb);
+    (void)sb8;
+    /* b0 and the slice bounds are wave-uniform: scalar loop and branches */
+    for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
+        real fv[kR][4], sv[kR][4];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            const int j = b0 + r * kWave * 4 + lane * 4;
+            if (b0 + r * kWave * 4 < Np) {
+                ldsLoad4(F + j, fv[r]);
+                if constexpr (FP8)
+                    fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(sb8 + j), sv[r]);
+                else
+                    ldsLoad4(sb + j, sv[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            const int j = b0 + r * kWave * 4 + lane * 4;
+            if (b0 + r * kWave * 4 < Np) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    fv[r][t] = fv[r][t] + d * sv[r][t];
+                ldsStore4(F + j, fv[r]);
+            }
+        }
+    }
+}
+
+/* The helper wave of a slice (float local-field sweep): waits for commands
+ * from the sweeping wave and applies the rows it hands over, F += d * row,
+ * with the same per-element operation the sweeping wave would use. */
+template <class real>
+__device__ void helperLoop(const SweepArgs<real> &a, unsigned char *p) {
+    const int Np = a.Np;
+    const int lane = threadIdx.x & (kWave - 1);
+    real *F = reinterpret_cast<real *>(p + sweepFixedBytes<real>());
+    uint32_t *ctl = helperCtl(a, p);
+    real *const buf[2] = {F + Np, reinterpret_cast<real *>(reinterpret_cast<unsigned char *>(ctl) - (size_t)Np * sizeof(real))};
+    for (;;) {
+        const uint32_t c = helpLoad(ctl);
+        if (c == kHelpIdle) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        if (c == kHelpExit)
+            break;
+        const uint32_t cmd = c - 2u;
+        addStagedRow<real, false>(F, buf[cmd >> 1], (cmd & 1u) ? (real)-2. : (real)2., Np, lane);
+        ldsWait(); /* the field stores are done */
+        helpStore(ctl, kHelpIdle);
+    }
 }
 
 /* One wave runs the whole sweep of slice y: N attempts, with its LDS region
  * p (sweepWaveBytes), the exp table and h already in LDS.  MULTI compiles the
  * multi-flip batches in (float local-field form only); without it the batch
  * ends at its first accepted flip. */
-template <class real, int MODE, bool MULTI = false>
+template <class real, int MODE, bool MULTI = false, bool HELPER = false>
 __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y, const uint64_t *expTab,
                                            const real *hS, unsigned char *p) {
 #pragma clang fp contract(off)
@@ -344,6 +450,15 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     };
     (void)stageBuf;
     (void)stageBufL;
+    /* Helper wave: the pending row is handed to the slice's second wave at
+     * the commit instead of being applied there, and the next row streams
+     * into the other of stage buffers 0 and 1.  The handed row must be in F
+     * before the next evaluation reads F (helpBusy: wait for kHelpIdle). */
+    constexpr bool kHelp = HELPER && kDefer && !FP8 && !kMultiCap && helperCapable<real, MODE>();
+    uint32_t *const ctl = reinterpret_cast<uint32_t *>(multiBase + stageBytes);
+    bool helpBusy = false;
+    (void)ctl;
+    (void)helpBusy;
 
     const long long yN = (long long)y * N;
     const int n0 = (y == 0) ? m - 1 : y - 1;
@@ -395,41 +510,8 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     real pendD[kMaxPend] = {0, 0, 0, 0};
     (void)nPend;
     (void)pendD;
-    /* F += d * (row in stage buffer b), four 256-element slices per round:
-     * all their LDS reads are issued before the first write, so the round
-     * trips overlap (Np is a multiple of 256; the tail round covers what is
-     * left) */
-    auto applyRow = [&](int b, real d) {
-        constexpr int kR = 4;
-        const real *sb = stageBuf(b);
-        const uint8_t *sb8 = reinterpret_cast<const uint8_t *>(sb);
-        (void)sb8;
-        /* b0 and the slice bounds are wave-uniform: scalar loop and branches */
-        for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
-            real fv[kR][4], sv[kR][4];
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {
-                const int j = b0 + r * kWave * 4 + lane * 4;
-                if (b0 + r * kWave * 4 < Np) {
-                    ldsLoad4(F + j, fv[r]);
-                    if constexpr (FP8)
-                        fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(sb8 + j), sv[r]);
-                    else
-                        ldsLoad4(sb + j, sv[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {
-                const int j = b0 + r * kWave * 4 + lane * 4;
-                if (b0 + r * kWave * 4 < Np) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        fv[r][t] = fv[r][t] + d * sv[r][t];
-                    ldsStore4(F + j, fv[r]);
-                }
-            }
-        }
-    };
+    /* F += d * (row in stage buffer b) */
+    auto applyRow = [&](int b, real d) { addStagedRow<real, FP8>(F, stageBuf(b), d, Np, lane); };
     auto applyPendingRow = [&]() {
         applyRow(curStage, rowD);
         rowPending = false;
@@ -753,6 +835,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         } else {
             /* decided under both outcomes of the pending attempt: if they
              * agree the decision is final, otherwise the lane waits */
+            if constexpr (kHelp) {
+                if (helpBusy) {
+                    while (helpLoad(ctl) != kHelpIdle)
+                        ;
+                    helpBusy = false;
+                }
+            }
             real f = F[s];
             if constexpr (kMultiCap) {
 #pragma unroll
@@ -950,6 +1039,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         if constexpr (kMultiCap) {
             if (nPend > 0)
                 applyPendingRows();
+        } else if constexpr (kHelp) {
+            if (rowPending) {
+                /* the row landed before this batch's evaluation (vmcnt) */
+                helpStore(ctl, helpApply(curStage, rowD < (real)0.));
+                helpBusy = true;
+                rowPending = false;
+            }
         } else if constexpr (kDefer) {
             if (rowPending)
                 applyPendingRow();
@@ -1044,8 +1140,14 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 const real *row = a.J + (long long)x * Np;
                 waveSync();
                 ldsWait(); /* this wave's reads of `stage` are done */
+                uint32_t dst = stageL;
+                if constexpr (kHelp) {
+                    /* the other buffer than the one the helper may be reading */
+                    curStage ^= 1;
+                    dst = stageBufL(curStage);
+                }
                 for (int j0 = 0; j0 < Np; j0 += kWave * 4)
-                    dmaDwordx4(row + j0 + lane * 4, stageL + j0 * 4);
+                    dmaDwordx4(row + j0 + lane * 4, dst + j0 * 4);
                 afterGroup += rowOps;
                 rowPending = true;
             } else if constexpr (!PLAIN) {
@@ -1094,6 +1196,11 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
 
     /* nothing may land in this region after the wave leaves it */
     vmWaitImm<0>();
+    if constexpr (kHelp) {
+        if (helpBusy)
+            while (helpLoad(ctl) != kHelpIdle)
+                ;
+    }
     if constexpr (kMultiCap) {
         if (nPend > 0 && !aborted) {
             applyPendingRows();
@@ -1120,6 +1227,8 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             if (b + lane < Nw)
                 a.Qbits[(long long)y * Nw + b + lane] = Qb[b + lane];
     }
+    if constexpr (kHelp)
+        helpStore(ctl, kHelpExit);
     if (lane == 0 && a.counters) {
         atomicAdd(&a.counters[0], (unsigned long long)nFlips);
         atomicAdd(&a.counters[1], (unsigned long long)nBatches);
@@ -1131,7 +1240,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
     }
 }
 
-template <class real, int MODE, bool MULTI = false>
+template <class real, int MODE, bool MULTI = false, bool HELPER = false>
 __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(SweepArgs<real> a, int slicesPerBlock,
                                                                                 int xcdRemap) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -1144,15 +1253,20 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
         int per = gridDim.x / 8;
         b = (blockIdx.x % 8) * per + blockIdx.x / 8;
     }
-    const int y = b * slicesPerBlock + w;
+    /* HELPER: one slice per block, wave 0 sweeps it and wave 1 applies rows */
+    const int y = HELPER ? b : b * slicesPerBlock + w;
 
-    /* LDS carve-up: exp table, h, then one region per wave */
+    /* LDS carve-up: exp table, h, then one region per slice */
     uint64_t *expTab = reinterpret_cast<uint64_t *>(lds);
     real *hS = reinterpret_cast<real *>(lds + kExpTabBytes);
-    const size_t waveBytes = sweepWaveBytes<real, MODE>(a.Np, a.ld8, MULTI ? a.multiBufs : 0);
-    unsigned char *p = lds + kExpTabBytes + (size_t)a.Np * sizeof(real) + waveBytes * w;
+    const size_t waveBytes = sweepWaveBytes<real, MODE>(a.Np, a.ld8, MULTI ? a.multiBufs : 0, HELPER);
+    unsigned char *p = lds + kExpTabBytes + (size_t)a.Np * sizeof(real) + waveBytes * (HELPER ? 0 : w);
     if (threadIdx.x < 32)
         expTab[threadIdx.x] = sq_exp2_tab32[threadIdx.x];
+    if constexpr (HELPER) {
+        if (threadIdx.x == 0)
+            *helperCtl(a, p) = kHelpIdle;
+    }
     {
         /* h by DMA, the block's waves taking 1 KiB pieces in turn (h holds Np
          * reals, a multiple of 1 KiB) */
@@ -1166,7 +1280,13 @@ __global__ __launch_bounds__(kMaxSlicesPerBlock *kWave) void denseSweepKernel(Sw
     __syncthreads();
     if (y >= a.m)
         return;
-    sweepSlice<real, MODE, MULTI>(a, y, expTab, hS, p);
+    if constexpr (HELPER) {
+        if (w == 1) {
+            helperLoop<real>(a, p);
+            return;
+        }
+    }
+    sweepSlice<real, MODE, MULTI, HELPER>(a, y, expTab, hS, p);
 }
 
 /* Many instances in one persistent launch: every wave takes (instance,
@@ -1240,13 +1360,14 @@ void fp8DecodeTable(float out[256]) {
 }
 
 namespace {
-size_t waveBytesFor(int Np, size_t realSize, bool plain, int fieldFormat, int ld8, int multiBufs = 0) {
+size_t waveBytesFor(int Np, size_t realSize, bool plain, int fieldFormat, int ld8, int multiBufs = 0,
+                    bool helper = false) {
     if (realSize == 8)
         return plain ? sweepWaveBytes<double, kModePlain>(Np, ld8) : sweepWaveBytes<double, kModeLocal>(Np, ld8);
     if (plain)
         return sweepWaveBytes<float, kModePlain>(Np, ld8);
     return fieldFormat == fieldFP8 ? sweepWaveBytes<float, kModeFP8>(Np, ld8)
-                                   : sweepWaveBytes<float, kModeLocal>(Np, ld8, multiBufs);
+                                   : sweepWaveBytes<float, kModeLocal>(Np, ld8, multiBufs, helper);
 }
 
 template <class real>
@@ -1261,9 +1382,15 @@ void *batchKernelFor(bool plain, int fieldFormat) {
 }  // namespace
 
 size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, int ld8,
-                          int multiBufs) {
+                          int multiBufs, bool helper) {
     return kExpTabBytes + (size_t)Np * realSize +
-           waveBytesFor(Np, realSize, plain, fieldFormat, ld8, multiBufs) * slicesPerBlock;
+           waveBytesFor(Np, realSize, plain, fieldFormat, ld8, multiBufs, helper) * slicesPerBlock;
+}
+
+bool denseSweepHelperFits(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat,
+                          size_t maxLds) {
+    return realSize == 4 && !plain && fieldFormat != fieldFP8 && slicesPerBlock == 1 &&
+           denseSweepLdsBytes(Np, 1, realSize, plain, fieldFormat, 0, 0, true) <= maxLds;
 }
 
 int denseSweepMultiBufs(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat, size_t maxLds) {
@@ -1356,9 +1483,15 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
     const bool fp8 = !plain && fieldFormat == fieldFP8;
     SQ_THROW_IF(fp8 && (sizeof(real) != 4 || !args.J8 || !args.rowScale || args.ld8 < args.Np || args.ld8 % 1024),
                 "fp8 sweep: needs a float solver and e4m3 rows with a 1 KiB-multiple stride");
+    /* the helper-wave kernel: float local-field form, single-flip batches,
+     * one slice per block */
+    const bool helper =
+        args.helperWave && sizeof(real) == 4 && !plain && !fp8 && args.multiBufs < 2 && slicesPerBlock == 1;
+    SQ_THROW_IF(args.helperWave && !helper && sizeof(real) == 4 && !plain && !fp8 && args.multiBufs < 2,
+                "helper-wave sweep: needs one slice per block");
     int nBlocks = divru(args.m, slicesPerBlock);
-    size_t lds =
-        denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain, fieldFormat, args.ld8, args.multiBufs);
+    size_t lds = denseSweepLdsBytes(args.Np, slicesPerBlock, sizeof(real), plain, fieldFormat, args.ld8,
+                                    args.multiBufs, helper);
     int xcdRemap = (nBlocks % 8 == 0 && nBlocks >= 16) ? 1 : 0;
 
     void *fn = plain ? (void *)denseSweepKernel<real, kModePlain> : (void *)denseSweepKernel<real, kModeLocal>;
@@ -1367,15 +1500,17 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
             fn = (void *)denseSweepKernel<real, kModeFP8>;
         else if (!plain && args.multiBufs >= 2)
             fn = (void *)denseSweepKernel<real, kModeLocal, true>;
+        else if (helper)
+            fn = (void *)denseSweepKernel<real, kModeLocal, false, true>;
     }
+    const int threads = helper ? 2 * kWave : slicesPerBlock * kWave;
     HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     SweepArgs<real> a = args;
     void *kargs[] = {(void *)&a, (void *)&slicesPerBlock, (void *)&xcdRemap};
-    if (allowDirect && fitsAtOnce(fn, slicesPerBlock * kWave, lds, nBlocks, numCUs))
-        HIP_CHECK(hipLaunchKernel(fn, dim3(nBlocks), dim3(slicesPerBlock * kWave), kargs, (unsigned)lds, stream));
+    if (allowDirect && fitsAtOnce(fn, threads, lds, nBlocks, numCUs))
+        HIP_CHECK(hipLaunchKernel(fn, dim3(nBlocks), dim3(threads), kargs, (unsigned)lds, stream));
     else
-        HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(nBlocks), dim3(slicesPerBlock * kWave), kargs, (unsigned)lds,
-                                             stream));
+        HIP_CHECK(hipLaunchCooperativeKernel(fn, dim3(nBlocks), dim3(threads), kargs, (unsigned)lds, stream));
 }
 
 template void denseSweep<float>(const SweepArgs<float> &, bool, int, int, bool, int, hipStream_t);

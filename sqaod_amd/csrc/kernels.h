@@ -266,6 +266,8 @@ struct SweepArgs {
                              * flips (>= 2), 0: every batch ends at its first flip */
     int multiMinCand;       /* with multiBufs >= 2: accepting lanes of a batch that make the next batch
                              * commit all its flips */
+    int helperWave;         /* float local-field form, single-flip batches, one slice per block: a second
+                             * wave per slice applies the deferred J rows to the fields */
 };
 
 /* multi_flip: "off", "auto" (the default: multi-flip batches in the first
@@ -288,7 +290,11 @@ void denseSweep(const SweepArgs<real> &args, bool plain, int fieldFormat, int sl
 /* Bytes of LDS one block of the local-field sweep needs (ld8: the fp8 row
  * stride, used when fieldFormat == fieldFP8). */
 size_t denseSweepLdsBytes(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat = fieldF32,
-                          int ld8 = 0, int multiBufs = 0);
+                          int ld8 = 0, int multiBufs = 0, bool helper = false);
+/* Whether the helper-wave sweep (SweepArgs::helperWave) applies and fits in
+ * maxLds bytes per block. */
+bool denseSweepHelperFits(int Np, int slicesPerBlock, size_t realSize, bool plain, int fieldFormat,
+                          size_t maxLds);
 /* Row stage buffers for multi-flip batches that fit beside S slices' regions
  * in maxLds bytes (0: none; at most 4, and (buffers - 1) rows of DMA ops within
  * the 63 outstanding memory operations a wave can count). */

@@ -244,6 +244,9 @@ private:
     int multiFlipMode_ = kMultiFlipAuto;
     int multiFlipMin_ = kMultiFlipDefaultMin;
     int sweepsSinceSpinsSet_ = 0;
+    /* single-flip float local-field sweeps with one slice per block: a second
+     * wave per slice applies the deferred J rows (where LDS allows) */
+    bool helperWave_ = true;
     int sysTraceCap_ = 0, sysTraceLen_ = 0;
     long long sysTraceSteps_ = 0;
     int mtWorkersSeeded_ = 0; /* generators in d_mt_ */

@@ -362,3 +362,51 @@ def test_multi_flip_engages_and_matches_single_flip(rocm, N, m):
     assert np.array_equal(runs["off"][0], runs["2"][0])
     assert runs["2"][1]["flips"] == runs["off"][1]["flips"]
     assert runs["2"][1]["batches"] < 0.8 * runs["off"][1]["batches"]
+
+
+@pytest.mark.parametrize("N,m,form,algo", [(128, 32, "fp32", "coloring"), (300, 7, "fp32", "coloring"),
+                                           (1000, 16, "bf16x2", "coloring"), (96, 1, "fp32", "sa_naive"),
+                                           (4096, 8, "fp32", "coloring"), (6000, 3, "bf16x2", "coloring")])
+def test_helper_wave_bit_exact(oracle, rocm, N, m, form, algo):
+    """helper_wave=on (the default): a second wave per slice adds the last
+    flip's J row to the LDS fields while the sweeping wave runs the next
+    batch's neighbour phase, the next row streaming into the other stage
+    buffer; the sweeping wave waits for it before reading a field.  With
+    multi_flip=off every sweep runs that kernel, and the trajectory is still
+    the serial CPU annealer's, bit for bit."""
+    W = dense_int_W(N, SEED_BASE + 5 * N + m)
+    ora, an = make_pair(oracle, rocm, W, m, seed=41 + m, algo=algo, field_precision=form, multi_flip="off",
+                        helper_wave="on")
+    assert an.get_preferences()["helper_wave"] == "on"
+    sch = schedule() if N <= 300 else schedule()[:30]
+    if algo == "sa_naive":
+        sch = list(np.geomspace(20.0, 0.05, 60))
+    for k, G in enumerate(sch):
+        beta = 0.0 if algo == "sa_naive" else BETA
+        ora.anneal_one_step(G, beta)
+        an.anneal_one_step(G, beta)
+        if k % 10 == 0 or k == len(sch) - 1:
+            qa, qo = q_of(an), ora.get_q()
+            assert np.array_equal(qa, qo), f"step {k}: {int(np.sum(qa != qo))} spins differ"
+    assert np.array_equal(an.get_E(), ora.calculate_E())
+
+
+def test_helper_wave_on_off_same_sweep(rocm):
+    """The helper wave changes who adds a row to the fields, not the sweep:
+    the same spins, flips and energies with it on and off (float normal W, so
+    the fields carry rounding that any change of operation order would
+    show).  Batch counts are not compared: where a batch ends depends on how
+    far the neighbour slices have got, which is timing."""
+    W = dense_normal_W(2048, SEED_BASE + 99)
+    runs = {}
+    for mode in ("off", "on"):
+        an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=32, helper_wave=mode)
+        an.seed(9)
+        an.prepare()
+        an.randomize_spin()
+        for G in schedule()[:12]:
+            an.anneal_one_step(G, BETA)
+        runs[mode] = (q_of(an), an.get_counters(), an.get_E())
+    assert np.array_equal(runs["off"][0], runs["on"][0])
+    assert runs["off"][1]["flips"] == runs["on"][1]["flips"]
+    assert np.array_equal(runs["off"][2], runs["on"][2])
