@@ -328,6 +328,12 @@ __device__ __forceinline__ uint32_t *helperCtl(const SweepArgs<real> &a, unsigne
     return reinterpret_cast<uint32_t *>(buf1 + stageBytes);
 }
 
+/* f + d * v for a field update: d is -+2 (fp8 rows: -+2 times the row's
+ * power-of-two scale) and v a J element, so d * v is exact and one fused
+ * multiply-add rounds exactly as the separate multiply and add do. */
+__device__ __forceinline__ float addScaled(float f, float d, float v) { return __builtin_fmaf(d, v, f); }
+__device__ __forceinline__ double addScaled(double f, double d, double v) { return __builtin_fma(d, v, f); }
+
 /* F += d * (staged row sb), four 256-element slices per round: all their
  * LDS reads are issued before the first write, so the round trips overlap
  * (Np is a multiple of 256; the tail round covers what is left).  FP8: sb
@@ -358,7 +364,7 @@ __device__ __forceinline__ void addStagedRow(real *F, const real *sb, real d, in
             if (b0 + r * kWave * 4 < Np) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    fv[r][t] = fv[r][t] + d * sv[r][t];
+                    fv[r][t] = addScaled(fv[r][t], d, sv[r][t]);
                 ldsStore4(F + j, fv[r]);
             }
         }
@@ -547,7 +553,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                         if (q < nPend)
 #pragma unroll
                             for (int t = 0; t < 4; ++t)
-                                fv[r][t] = fv[r][t] + pendD[q] * sv[r][q][t];
+                                fv[r][t] = addScaled(fv[r][t], pendD[q], sv[r][q][t]);
                     ldsStore4(F + j, fv[r]);
                 }
             }
@@ -847,10 +853,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
 #pragma unroll
                 for (int q = 0; q < kMaxPend; ++q)
                     if (q < nPend)
-                        f = f + pendD[q] * pendCorr[q];
+                        f = addScaled(f, pendD[q], pendCorr[q]);
             } else if constexpr (kDefer) {
                 if (rowPending)
-                    f = f + rowD * rowCorr;
+                    f = addScaled(f, rowD, rowCorr);
             }
             accLocal = decide(f);
             if (hypNb >= 0 && decideWith(f, nsumAlt) != accLocal)
@@ -935,7 +941,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 if (lane == fa)
                     dFlip = da;
                 if (lane > fa) {
-                    fLane = fLane + da * sub[fa * kWave + lane];
+                    fLane = addScaled(fLane, da, sub[fa * kWave + lane]);
                     if (s == xa)
                         q = -q;
                     accLocal = decide(fLane);
@@ -1172,7 +1178,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                             ldsLoad4(F + j, fv);
 #pragma unroll
                             for (int t = 0; t < 4; ++t)
-                                fv[t] = fv[t] + d * v[c4][t];
+                                fv[t] = addScaled(fv[t], d, v[c4][t]);
                             ldsStore4(F + j, fv);
                         }
                     }
