@@ -905,10 +905,15 @@ void DenseGraphAnnealer<real>::annealOneStep(real G, real beta) {
         if (a.multiBufs >= 2)
             sweepFormat = f;
     }
-    a.helperWave = (helperWave_ && a.multiBufs < 2 &&
-                    denseSweepHelperFits(Np_, S, sizeof(real), plainSweep_, sweepFormat, dev_->maxLdsPerBlock()))
-                       ? 1
-                       : 0;
+    /* the helper-wave sweep where it fits; the fp8 form's runs on the float
+     * copy of J' as well (the same row values, -+2 J'[x][s]) */
+    if (helperWave_ && a.multiBufs < 2) {
+        const int f = sweepFormat == fieldFP8 ? fieldF32 : sweepFormat;
+        if (denseSweepHelperFits(Np_, S, sizeof(real), plainSweep_, f, dev_->maxLdsPerBlock())) {
+            a.helperWave = 1;
+            sweepFormat = f;
+        }
+    }
     ++sweepsSinceSpinsSet_;
     if (profile_)
         phaseBegin(2, pe);

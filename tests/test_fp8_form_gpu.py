@@ -80,11 +80,15 @@ def test_form_energy_error(oracle, rocm):
     assert an32.get_form_energy_error() == 0.0
 
 
+@pytest.mark.parametrize("helper", ["on", "off"])
 @pytest.mark.parametrize("N,m", [(128, 32), (1100, 6), (300, 7)])
-def test_bit_exact_when_J_is_exact_in_e4m3(oracle, rocm, N, m):
+def test_bit_exact_when_J_is_exact_in_e4m3(oracle, rocm, N, m, helper):
     """Integer W in [-8, 8] (J quarter-integral, |J| <= 2, exact in e4m3):
     the fp8 form's spins equal the serial CPU annealer's (MT19937 stream
-    injected) after every 10th step, and the energies at the end."""
+    injected) after every 10th step, and the energies at the end.  With
+    helper_wave=off the single-flip sweeps stream the e4m3 rows and their
+    scales; with it on they run the helper-wave kernel on the float copy of
+    J'."""
     W = dense_int_W(N, SEED_BASE + 5 * N + m)
     ora = oracle.DenseGraphAnnealer(np.float32, 1)
     ora.seed(1000 + N)
@@ -93,7 +97,8 @@ def test_bit_exact_when_J_is_exact_in_e4m3(oracle, rocm, N, m):
     ora.set_algorithm(3)
     ora.prepare()
     ora.randomize_spin()
-    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m, rng="mt19937", field_precision="fp8")
+    an = rocm.dense_graph_annealer(W, dtype=np.float32, n_trotters=m, rng="mt19937", field_precision="fp8",
+                                   helper_wave=helper)
     an.seed(1000 + N)
     an.prepare()
     an.randomize_spin()
