@@ -601,7 +601,7 @@ void DenseGraphAnnealer<real>::prepare() {
     ensureEnergyPlanes();
     d_spinDot_.allocate(1);
     d_abort_.allocate(1);
-    d_counters_.allocate(16); /* [0..2] public, [3..13] optional per-phase cycles */
+    d_counters_.allocate(16); /* [0..2] public, [3..15] optional per-phase cycles */
     HIP_CHECK(hipMemsetAsync(d_abort_.ptr, 0, sizeof(int), dev_->stream()));
     HIP_CHECK(hipMemsetAsync(d_counters_.ptr, 0, 16 * sizeof(unsigned long long), dev_->stream()));
     h_abort_.allocate(1, hipHostMallocCoherent | hipHostMallocMapped);

@@ -35,6 +35,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 namespace sqaod_rocm {
 
@@ -47,7 +48,7 @@ constexpr int kMaxSlicesPerBlock = 4; /* one wave per SIMD, so a wave may hold u
 constexpr size_t kExpTabBytes = 32 * sizeof(uint64_t); /* LDS copy of the exp table */
 
 /* Optional per-phase cycle counters (build with -DSQAOD_SWEEP_CYCLES): shader
- * clock cycles spent per phase, summed into SweepArgs::counters[3..13]. */
+ * clock cycles spent per phase, summed into SweepArgs::counters[3..15]. */
 #ifdef SQAOD_SWEEP_CYCLES
 #define SQ_CYC(v) const unsigned long long v = __builtin_amdgcn_s_memtime()
 #define SQ_CYC_ADD(acc, a, b) (acc) += (b) - (a)
@@ -270,6 +271,7 @@ __host__ __device__ constexpr bool multiFlipCapable() {
 /* Helper wave (float local-field sweep, one slice per workgroup): a second
  * row stage buffer and the command word the two waves exchange. */
 constexpr size_t kHelperCtlBytes = 256;
+constexpr int kHelperRounds = 4; /* the helper's row add: 256-element slices in flight */
 
 template <class real, int MODE>
 __host__ __device__ constexpr bool helperCapable() {
@@ -334,41 +336,80 @@ __device__ __forceinline__ uint32_t *helperCtl(const SweepArgs<real> &a, unsigne
 __device__ __forceinline__ float addScaled(float f, float d, float v) { return __builtin_fmaf(d, v, f); }
 __device__ __forceinline__ double addScaled(double f, double d, double v) { return __builtin_fma(d, v, f); }
 
-/* F += d * (staged row sb), four 256-element slices per round: all their
- * LDS reads are issued before the first write, so the round trips overlap
- * (Np is a multiple of 256; the tail round covers what is left).  FP8: sb
- * holds e4m3 bytes. */
-template <class real, bool FP8>
-__device__ __forceinline__ void addStagedRow(real *F, const real *sb, real d, int Np, int lane) {
+/* F[b0 ..] += d * sb[b0 ..] for R consecutive 256-element slices: all their
+ * LDS reads are issued before the first write, so the round trips overlap.
+ * FP8: sb holds e4m3 bytes. */
+template <class real, bool FP8, int R>
+__device__ __forceinline__ void addStagedSlices(real *F, const real *sb, real d, int b0, int lane) {
 #pragma clang fp contract(off)
-    constexpr int kR = 4;
     const uint8_t *sb8 = reinterpret_cast<const uint8_t *>(sb);
     (void)sb8;
-    /* b0 and the slice bounds are wave-uniform: scalar loop and branches */
-    for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
-        real fv[kR][4], sv[kR][4];
+    real fv[R][4], sv[R][4];
 #pragma unroll
-        for (int r = 0; r < kR; ++r) {
+    for (int r = 0; r < R; ++r) {
+        const int j = b0 + r * kWave * 4 + lane * 4;
+        ldsLoad4(F + j, fv[r]);
+        if constexpr (FP8)
+            fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(sb8 + j), sv[r]);
+        else
+            ldsLoad4(sb + j, sv[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = b0 + r * kWave * 4 + lane * 4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            fv[r][t] = addScaled(fv[r][t], d, sv[r][t]);
+        ldsStore4(F + j, fv[r]);
+    }
+}
+
+/* F += d * (staged row sb): rounds of kR 256-element slices, then single
+ * slices for the rest (Np is a multiple of 256).  The rounds carry no
+ * per-slice bounds checks, so their reads go out back to back. */
+template <class real, bool FP8, int kR = 4>
+__device__ __forceinline__ void addStagedRow(real *F, const real *sb, real d, int Np, int lane) {
+    int b0 = 0;
+    /* b0 and the bounds are wave-uniform: scalar loops */
+    for (; b0 + kR * kWave * 4 <= Np; b0 += kR * kWave * 4)
+        addStagedSlices<real, FP8, kR>(F, sb, d, b0, lane);
+    for (; b0 < Np; b0 += kWave * 4)
+        addStagedSlices<real, FP8, 1>(F, sb, d, b0, lane);
+}
+
+/* F += d[0] * row 0, then d[1] * row 1, ... (NR rows, in that order per
+ * element), reading and writing F once for all of them: rounds of two
+ * 256-element slices, then a single slice if one is left. */
+template <class real, int NR>
+__device__ __forceinline__ void addStagedRows(real *F, const real *const *sb, const real *d, int Np, int lane) {
+#pragma clang fp contract(off)
+    auto slices = [&](int b0, auto rTag) {
+        constexpr int R = decltype(rTag)::value;
+        real fv[R][4], sv[R][NR][4];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
             const int j = b0 + r * kWave * 4 + lane * 4;
-            if (b0 + r * kWave * 4 < Np) {
-                ldsLoad4(F + j, fv[r]);
-                if constexpr (FP8)
-                    fp8x4ToFloat(*reinterpret_cast<const uint32_t *>(sb8 + j), sv[r]);
-                else
-                    ldsLoad4(sb + j, sv[r]);
-            }
+            ldsLoad4(F + j, fv[r]);
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+                ldsLoad4(sb[q] + j, sv[r][q]);
         }
 #pragma unroll
-        for (int r = 0; r < kR; ++r) {
+        for (int r = 0; r < R; ++r) {
             const int j = b0 + r * kWave * 4 + lane * 4;
-            if (b0 + r * kWave * 4 < Np) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
-                    fv[r][t] = addScaled(fv[r][t], d, sv[r][t]);
-                ldsStore4(F + j, fv[r]);
-            }
+                    fv[r][t] = addScaled(fv[r][t], d[q], sv[r][q][t]);
+            ldsStore4(F + j, fv[r]);
         }
-    }
+    };
+    int b0 = 0;
+    for (; b0 + 2 * kWave * 4 <= Np; b0 += 2 * kWave * 4)
+        slices(b0, std::integral_constant<int, 2>());
+    if (b0 < Np)
+        slices(b0, std::integral_constant<int, 1>());
 }
 
 /* The helper wave of a slice (float local-field sweep): waits for commands
@@ -390,7 +431,7 @@ __device__ void helperLoop(const SweepArgs<real> &a, unsigned char *p) {
         if (c == kHelpExit)
             break;
         const uint32_t cmd = c - 2u;
-        addStagedRow<real, false>(F, buf[cmd >> 1], (cmd & 1u) ? (real)-2. : (real)2., Np, lane);
+        addStagedRow<real, false, kHelperRounds>(F, buf[cmd >> 1], (cmd & 1u) ? (real)-2. : (real)2., Np, lane);
         ldsWait(); /* the field stores are done */
         helpStore(ctl, kHelpIdle);
     }
@@ -530,34 +571,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
             nPend = 0;
             return;
         }
-        constexpr int kR = 2;
-        for (int b0 = 0; b0 < Np; b0 += kR * kWave * 4) {
-            real fv[kR][4], sv[kR][kMaxPend][4];
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {
-                const int j = b0 + r * kWave * 4 + lane * 4;
-                if (b0 + r * kWave * 4 < Np) {
-                    ldsLoad4(F + j, fv[r]);
-#pragma unroll
-                    for (int q = 0; q < kMaxPend; ++q)
-                        if (q < nPend)
-                            ldsLoad4(stageBuf(q) + j, sv[r][q]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < kR; ++r) {
-                const int j = b0 + r * kWave * 4 + lane * 4;
-                if (b0 + r * kWave * 4 < Np) {
-#pragma unroll
-                    for (int q = 0; q < kMaxPend; ++q)
-                        if (q < nPend)
-#pragma unroll
-                            for (int t = 0; t < 4; ++t)
-                                fv[r][t] = addScaled(fv[r][t], pendD[q], sv[r][q][t]);
-                    ldsStore4(F + j, fv[r]);
-                }
-            }
-        }
+        const real *const sbs[kMaxPend] = {stageBuf(0), stageBuf(1), stageBuf(2), stageBuf(3)};
+        if (nPend == 2)
+            addStagedRows<real, 2>(F, sbs, pendD, Np, lane);
+        else if (nPend == 3)
+            addStagedRows<real, 3>(F, sbs, pendD, Np, lane);
+        else
+            addStagedRows<real, 4>(F, sbs, pendD, Np, lane);
         nPend = 0;
     };
     (void)applyPendingRows;
@@ -623,8 +643,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
 
     /* load wait, neighbours, evaluate, commit, row, blocked; within neighbours:
      * replay and registration, row wait, prefetch issue; within the row:
-     * the pending row's application, the next row's DMA issue */
-    unsigned long long cyc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+     * the pending row's application, the next row's DMA issue; within the
+     * evaluation: the wait for the helper wave; multi-flip commits: the
+     * waits for rows applied at once */
+    unsigned long long cyc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     (void)cyc;
     /* multi mode for the next batch: chosen from how many flips the last
      * batch committed (several, or one within its first lanes) */
@@ -843,9 +865,12 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
              * agree the decision is final, otherwise the lane waits */
             if constexpr (kHelp) {
                 if (helpBusy) {
+                    SQ_CYC(h0);
                     while (helpLoad(ctl) != kHelpIdle)
                         ;
                     helpBusy = false;
+                    SQ_CYC(h1);
+                    SQ_CYC_ADD(cyc[11], h0, h1);
                 }
             }
             real f = F[s];
@@ -1116,7 +1141,10 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 while (issued < nImm && issued < nBufs)
                     issueNext();
                 while (applied < nImm) {
+                    SQ_CYC(w0);
                     vmWait((issued - 1 - applied) * rowOps);
+                    SQ_CYC(w1);
+                    SQ_CYC_ADD(cyc[12], w0, w1);
                     const int fl = uni(firstLane(applyMask));
                     applyMask &= applyMask - 1;
                     applyRow(applyBuf, readLaneF(dFlip, fl));
@@ -1240,7 +1268,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         atomicAdd(&a.counters[1], (unsigned long long)nBatches);
         atomicAdd(&a.counters[2], (unsigned long long)nBlocked);
 #ifdef SQAOD_SWEEP_CYCLES
-        for (int t = 0; t < 11; ++t)
+        for (int t = 0; t < 13; ++t)
             atomicAdd(&a.counters[3 + t], cyc[t]);
 #endif
     }
