@@ -113,6 +113,29 @@ __device__ __forceinline__ void dmaDwordx4(const void *src, uint32_t lds) {
     const uint32_t m = __builtin_amdgcn_readfirstlane(lds);
     asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(src), "s"(m) : "memory", "m0");
 }
+/* n consecutive 1 KiB DMA ops (16 B per lane): op i reads src + i KiB (src:
+ * this lane's address for op 0) and lands at LDS byte address lds + i KiB.
+ * Groups of four set M0 once and step it by 1 KiB (a wait state after each
+ * M0 write before the DMA reads it). */
+__device__ __forceinline__ void dmaRunx4(const unsigned char *src, uint32_t lds, int n) {
+    int i = 0;
+    for (; i + 4 <= n; i += 4) { /* n is wave-uniform: a scalar loop */
+        const uint32_t m = __builtin_amdgcn_readfirstlane(lds + (uint32_t)i * 1024u);
+        const unsigned char *p = src + (size_t)i * 1024;
+        asm volatile("s_mov_b32 m0, %4\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %0, off\n\t"
+                     "s_add_u32 m0, m0, 0x400\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %1, off\n\t"
+                     "s_add_u32 m0, m0, 0x400\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %2, off\n\t"
+                     "s_add_u32 m0, m0, 0x400\n\ts_nop 0\n\t"
+                     "global_load_lds_dwordx4 %3, off" ::"v"(p),
+                     "v"(p + 1024), "v"(p + 2048), "v"(p + 3072), "s"(m)
+                     : "memory", "m0", "scc");
+    }
+    for (; i < n; ++i)
+        dmaDwordx4(src + (size_t)i * 1024, lds + (uint32_t)i * 1024u);
+}
 #pragma clang diagnostic pop
 
 /* wait until at most C vector-memory ops are outstanding */
@@ -1101,8 +1124,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 const uint8_t *row = a.J8 + (long long)x * a.ld8;
                 waveSync();
                 ldsWait(); /* this wave's reads of `stage` and the scale are done */
-                for (int j0 = 0; j0 < a.ld8; j0 += kWave * 16)
-                    dmaDwordx4(row + j0 + lane * 16, stageL + j0);
+                dmaRunx4(row + lane * 16, stageL, a.ld8 / (kWave * 16));
                 if (lane == 0)
                     dmaDword<false>(a.rowScale + x, scaleL);
                 afterGroup += rowOps;
@@ -1131,9 +1153,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                     issueMask &= issueMask - 1;
                     const uint32_t xr = (uint32_t)__builtin_amdgcn_readlane((int)s, fl);
                     const real *row = a.J + (long long)xr * Np;
-                    const uint32_t dst = stageBufL(issueBuf);
-                    for (int j0 = 0; j0 < Np; j0 += kWave * 4)
-                        dmaDwordx4(row + j0 + lane * 4, dst + j0 * 4);
+                    dmaRunx4(reinterpret_cast<const unsigned char *>(row + lane * 4), stageBufL(issueBuf), rowOps);
                     afterGroup += rowOps;
                     ++issued;
                     issueBuf = (issueBuf + 1 == nBufs) ? 0 : issueBuf + 1;
@@ -1180,8 +1200,7 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                     curStage ^= 1;
                     dst = stageBufL(curStage);
                 }
-                for (int j0 = 0; j0 < Np; j0 += kWave * 4)
-                    dmaDwordx4(row + j0 + lane * 4, dst + j0 * 4);
+                dmaRunx4(reinterpret_cast<const unsigned char *>(row + lane * 4), dst, rowOps);
                 afterGroup += rowOps;
                 rowPending = true;
             } else if constexpr (!PLAIN) {
