@@ -88,6 +88,9 @@ int sqaod_rocm_dg_annealer_get_problem_size(sqaod_rocm_dg_annealer *a, int *N);
  * number of) accepting lanes; "auto" does so in the first 3 sweeps after the
  * spins were set, when most attempts are accepted; the results are the same
  * in every mode),
+ * "helper_wave" ("on" | "off": single-flip float local-field sweeps with one
+ * slice per workgroup run a second wave per slice that applies the deferred
+ * J rows to the fields; the results are the same either way),
  * "slices_per_block" (int, 0 = auto), "sweep_launch" ("auto" | "cooperative"),
  * "refresh_interval" (int, steps between
  * field recomputes; 0 = only after the spins were set), "refresh_engine"
