@@ -136,6 +136,23 @@ __device__ __forceinline__ void dmaRunx4(const unsigned char *src, uint32_t lds,
     for (; i < n; ++i)
         dmaDwordx4(src + (size_t)i * 1024, lds + (uint32_t)i * 1024u);
 }
+/* A neighbour's two 128-entry windows when all their entries lie in its row:
+ * the decisions (coherent loads) at LDS byte address lds and the sites 1 KiB
+ * above it, 256 B per op, four ops on one M0 chain.  d, x: this lane's entry
+ * 0 address in each stream. */
+__device__ __forceinline__ void dmaNeighbourWindows(const uint32_t *d, const uint32_t *x, uint32_t lds) {
+    const uint32_t m = __builtin_amdgcn_readfirstlane(lds);
+    asm volatile("s_mov_b32 m0, %4\n\ts_nop 0\n\t"
+                 "global_load_lds_dword %0, off sc1\n\t"
+                 "s_add_u32 m0, m0, 0x100\n\ts_nop 0\n\t"
+                 "global_load_lds_dword %1, off sc1\n\t"
+                 "s_add_u32 m0, m0, 0x300\n\ts_nop 0\n\t"
+                 "global_load_lds_dword %2, off\n\t"
+                 "s_add_u32 m0, m0, 0x100\n\ts_nop 0\n\t"
+                 "global_load_lds_dword %3, off" ::"v"(d),
+                 "v"(d + 64), "v"(x), "v"(x + 64), "s"(m)
+                 : "memory", "m0", "scc");
+}
 #pragma clang diagnostic pop
 
 /* wait until at most C vector-memory ops are outstanding */
@@ -640,8 +657,13 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
         const long long nN = (long long)nbr[nb] * N;
         const int base = uni(applied[nb]);
         nbase[nb] = base;
-        dmaEntries<4, true, kWin>(a.D + nN + base, winDL + nb * kWin * 4, base, N);
-        dmaEntries<4, false, kWin>(a.X + nN + base, winSL + nb * kWin * 4, base, N);
+        if (base + kWin <= N) {
+            static_assert(kWin == 128, "two 64-lane ops per window");
+            dmaNeighbourWindows(a.D + nN + base + lane, a.X + nN + base + lane, winDL + nb * kWin * 4);
+        } else {
+            dmaEntries<4, true, kWin>(a.D + nN + base, winDL + nb * kWin * 4, base, N);
+            dmaEntries<4, false, kWin>(a.X + nN + base, winSL + nb * kWin * 4, base, N);
+        }
     };
     /* waitReads: wait for this wave's outstanding LDS reads first.  The batch
      * loop passes false: by then it has used every window entry it read (the
