@@ -1182,19 +1182,42 @@ __device__ __forceinline__ void sweepSlice(const SweepArgs<real> &a, const int y
                 };
                 while (issued < nImm && issued < nBufs)
                     issueNext();
+                /* rows land in issue order; two landed rows are added in one
+                 * pass over F (each element takes them in flip order), while
+                 * the next ones stream in */
                 while (applied < nImm) {
-                    SQ_CYC(w0);
-                    vmWait((issued - 1 - applied) * rowOps);
-                    SQ_CYC(w1);
-                    SQ_CYC_ADD(cyc[12], w0, w1);
-                    const int fl = uni(firstLane(applyMask));
-                    applyMask &= applyMask - 1;
-                    applyRow(applyBuf, readLaneF(dFlip, fl));
-                    ++applied;
-                    applyBuf = (applyBuf + 1 == nBufs) ? 0 : applyBuf + 1;
+                    if (applied + 1 < nImm) {
+                        /* rows applied and applied + 1 (issued >= applied + 2:
+                         * nBufs >= 2) */
+                        SQ_CYC(w0);
+                        vmWait((issued - 2 - applied) * rowOps);
+                        SQ_CYC(w1);
+                        SQ_CYC_ADD(cyc[12], w0, w1);
+                        const int fl0 = uni(firstLane(applyMask));
+                        applyMask &= applyMask - 1;
+                        const int fl1 = uni(firstLane(applyMask));
+                        applyMask &= applyMask - 1;
+                        const int b1 = (applyBuf + 1 == nBufs) ? 0 : applyBuf + 1;
+                        const real *const sbs[2] = {stageBuf(applyBuf), stageBuf(b1)};
+                        const real ds[2] = {readLaneF(dFlip, fl0), readLaneF(dFlip, fl1)};
+                        addStagedRows<real, 2>(F, sbs, ds, Np, lane);
+                        applied += 2;
+                        applyBuf = (b1 + 1 == nBufs) ? 0 : b1 + 1;
+                    } else {
+                        SQ_CYC(w0);
+                        vmWait((issued - 1 - applied) * rowOps);
+                        SQ_CYC(w1);
+                        SQ_CYC_ADD(cyc[12], w0, w1);
+                        const int fl = uni(firstLane(applyMask));
+                        applyMask &= applyMask - 1;
+                        applyRow(applyBuf, readLaneF(dFlip, fl));
+                        ++applied;
+                        applyBuf = (applyBuf + 1 == nBufs) ? 0 : applyBuf + 1;
+                    }
                     if (issued < nImm) {
-                        ldsWait(); /* the applied buffer's reads are done */
-                        issueNext();
+                        ldsWait(); /* the applied buffers' reads are done */
+                        while (issued < nImm && issued < applied + nBufs)
+                            issueNext();
                     }
                 }
                 ldsWait(); /* the applied buffers' reads are done */
