@@ -418,8 +418,8 @@ __device__ __forceinline__ void addStagedRow(real *F, const real *sb, real d, in
 }
 
 /* F += d[0] * row 0, then d[1] * row 1, ... (NR rows, in that order per
- * element), reading and writing F once for all of them: rounds of two
- * 256-element slices, then a single slice if one is left. */
+ * element), reading and writing F once for all of them: rounds of RR
+ * 256-element slices, then single slices for the rest. */
 template <class real, int NR>
 __device__ __forceinline__ void addStagedRows(real *F, const real *const *sb, const real *d, int Np, int lane) {
 #pragma clang fp contract(off)
@@ -445,10 +445,11 @@ __device__ __forceinline__ void addStagedRows(real *F, const real *const *sb, co
             ldsStore4(F + j, fv[r]);
         }
     };
+    constexpr int RR = NR <= 2 ? 4 : 2; /* slices per round: (1 + NR) RR reads in flight */
     int b0 = 0;
-    for (; b0 + 2 * kWave * 4 <= Np; b0 += 2 * kWave * 4)
-        slices(b0, std::integral_constant<int, 2>());
-    if (b0 < Np)
+    for (; b0 + RR * kWave * 4 <= Np; b0 += RR * kWave * 4)
+        slices(b0, std::integral_constant<int, RR>());
+    for (; b0 < Np; b0 += kWave * 4)
         slices(b0, std::integral_constant<int, 1>());
 }
 
