@@ -410,3 +410,19 @@ def test_helper_wave_on_off_same_sweep(rocm):
     assert np.array_equal(runs["off"][0], runs["on"][0])
     assert runs["off"][1]["flips"] == runs["on"][1]["flips"]
     assert np.array_equal(runs["off"][2], runs["on"][2])
+
+
+@pytest.mark.parametrize("spb", [2, 4])
+def test_helper_wave_with_several_slices_per_block(oracle, rocm, spb):
+    """The helper wave needs one slice per workgroup: with slices_per_block
+    2 or 4 and helper_wave=on the sweep runs the one-wave kernel, still the
+    serial CPU annealer's trajectory."""
+    N, m = 300, 7
+    W = dense_int_W(N, SEED_BASE + 17 * spb)
+    ora, an = make_pair(oracle, rocm, W, m, seed=61 + spb, slices_per_block=spb, helper_wave="on")
+    assert int(an.get_preferences()["slices_per_block"]) == spb
+    for k, G in enumerate(schedule()[:30]):
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+    assert np.array_equal(q_of(an), ora.get_q())
+    assert np.array_equal(an.get_E(), ora.calculate_E())
