@@ -426,3 +426,17 @@ def test_helper_wave_with_several_slices_per_block(oracle, rocm, spb):
         an.anneal_one_step(G, BETA)
     assert np.array_equal(q_of(an), ora.get_q())
     assert np.array_equal(an.get_E(), ora.calculate_E())
+
+
+def test_helper_wave_near_the_lds_limit(oracle, rocm):
+    """N = 8192: fields, two row stage buffers, h and the pending table fill
+    most of a workgroup's 160 KiB of LDS, and the helper-wave kernel still
+    runs the serial CPU annealer's trajectory."""
+    N, m = 8192, 2
+    W = dense_int_W(N, SEED_BASE + 8192)
+    ora, an = make_pair(oracle, rocm, W, m, seed=73, multi_flip="off", helper_wave="on")
+    for G in schedule()[:8]:
+        ora.anneal_one_step(G, BETA)
+        an.anneal_one_step(G, BETA)
+    assert np.array_equal(q_of(an), ora.get_q())
+    assert np.array_equal(an.get_E(), ora.calculate_E())
